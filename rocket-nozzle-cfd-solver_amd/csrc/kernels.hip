@@ -1,0 +1,1001 @@
+// kernels.hip — fused per-step kernels of the MI355X nozzle solver (gfx950, wave64).
+//
+// One solver step is two (three with the spray) launches instead of the
+// reference's ~21 draws (solver.js:849-891):
+//   k_stage<1>   LDS tile of primitives with a 2-cell ghost halo (FS_PRIM) ->
+//                each x/r face flux computed once (MUSCL + HLLC/HLL, FS_FLUX) ->
+//                merged-group finite-volume update U1 = U0 + dt L(U0) (FS_UPDATE).
+//   k_spray      parcel push, evaporation, break-up; deposits its release with
+//                float atomics into the host cell (FS_SPRAY_UPDATE + FS_SPRAY_DEP).
+//   k_stage<2>   the same flux/update on U1 giving U^{n+1} = (U0 + U1 + dt L(U1))/2,
+//                fused with the viscous / wall / afterburning split step
+//                (FS_VCELL, FS_VFLUX, FS_SOURCE), the spray source apply
+//                (FS_SPRAY_APPLY) and the CFL wave-speed max of the new state
+//                (FS_REDUCE1/2). U^{n+1} overwrites U0 in place: every group's
+//                old value is read and rewritten by its host cell's thread only.
+// The time step of the next step is taken by the first launch of that step from
+// the wave-speed max and the previous step info (FS_STEPINFO), so dt never
+// leaves the device. Launches carry a table of instances: one launch advances
+// every resident solve (batched serving uses the same kernels).
+#include "nz_device.h"
+
+namespace nzk {
+
+// ------------------------------------------------------------------------------
+// instance lookup for a flattened tile index (instances sorted by tile0)
+__device__ __forceinline__ int find_inst(const DevInst* __restrict__ I, int n, int t) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int m = (lo + hi + 1) >> 1;
+    if (I[m].tile0 <= t) lo = m; else hi = m - 1;
+  }
+  return lo;
+}
+
+// FS_STEPINFO (solver.js:404-409): dt from the wave-speed max of the current state
+__device__ __forceinline__ void step_info(const Consts& c, const InstState& s, float* cur) {
+  const float w = fmaxf(c.wFloor, __uint_as_float(s.wsnap));
+  float dt = fminf(c.cfl / w, c.dtMax);
+  if (s.last[0] > 0.f) dt = fminf(dt, s.last[0] * 1.2f);
+  cur[0] = dt;
+  cur[1] = s.last[2];
+  cur[2] = s.last[2] + dt;
+  cur[3] = w;
+}
+
+template <int NT>
+__device__ __forceinline__ float block_max(float v, float* red) {
+  for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if (lane == 0) red[wv] = v;
+  __syncthreads();
+  float m = 0.f;
+  if (threadIdx.x == 0) {
+    for (int k = 0; k < NT / 64; ++k) m = fmaxf(m, red[k]);
+  }
+  return m;  // valid on thread 0
+}
+template <int NT>
+__device__ __forceinline__ double block_sum(double v, double* red) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if (lane == 0) red[wv] = v;
+  __syncthreads();
+  double m = 0.0;
+  if (threadIdx.x == 0) {
+    for (int k = 0; k < NT / 64; ++k) m += red[k];
+  }
+  return m;
+}
+
+// ------------------------------------------------------------------------------
+// global-memory evaluation of one cell's own net inviscid flux (r-weighted face
+// areas), used for merged-group members that lie outside the LDS tile
+template <typename R>
+__device__ void own_flux_global(const DevInst& I, const R* U, R t, int ci, int cj, R* L, R* LY, R& LO) {
+  const int Nx = I.Nx;
+  const Consts& c = I.c;
+  // faces: west (ci, cj), east (ci+1, cj) in x; south (ci, cj), north (ci, cj+1) in r
+  for (int f = 0; f < 4; ++f) {
+    const bool dirR = f >= 2;
+    const int fi = ci + (f == 1 ? 1 : 0), fj = cj + (f == 3 ? 1 : 0);
+    const float area = dirR ? I.ar[(long long)fj * Nx + fi] : I.ax[(long long)fj * (Nx + 1) + fi];
+    if (area == 0.f || (dirR && fj == 0)) continue;
+    const int dx = dirR ? 0 : 1, dy = dirR ? 1 : 0, tx = dirR ? 1 : 0, ty = dirR ? 0 : 1;
+    Prim<R> c0 = prim_at<R>(I, U, fi - 2 * dx, fj - 2 * dy, t);
+    Prim<R> c1 = prim_at<R>(I, U, fi - dx, fj - dy, t);
+    Prim<R> c2 = prim_at<R>(I, U, fi, fj, t);
+    Prim<R> c3 = prim_at<R>(I, U, fi + dx, fj + dy, t);
+    const bool s1 = c1.r < R(0), s2 = c2.r < R(0);
+    if (s1 && s2) continue;
+    R hllw = R(0);
+    if (c.hllBlend > 0.f && !(s1 || s2)) {
+      const Prim<R> a1 = prim_at<R>(I, U, fi - dx + tx, fj - dy + ty, t), b1 = prim_at<R>(I, U, fi - dx - tx, fj - dy - ty, t);
+      const Prim<R> a2 = prim_at<R>(I, U, fi + tx, fj + ty, t), b2 = prim_at<R>(I, U, fi - tx, fj - ty, t);
+      hllw = hll_weight<R>(c1.p, c2.p, a1.r < R(0) ? c1.p : a1.p, b1.r < R(0) ? c1.p : b1.p,
+                           a2.r < R(0) ? c2.p : a2.p, b2.r < R(0) ? c2.p : b2.p, R(c.hllBlend));
+    }
+    if (dirR) {
+      R s;
+      s = c0.u; c0.u = c0.v; c0.v = s;
+      s = c1.u; c1.u = c1.v; c1.v = s;
+      s = c2.u; c2.u = c2.v; c2.v = s;
+      s = c3.u; c3.u = c3.v; c3.v = s;
+    }
+    R F[4], FY[4], FO;
+    face_flux<R>(c0, c1, c2, c3, hllw, c.order2 && area > 0.f, F, FY, FO);
+    if (dirR) { const R s = F[1]; F[1] = F[2]; F[2] = s; }
+    const R a = R(fabsf(area));
+    const R sg = (f == 1 || f == 3) ? R(-1) : R(1);  // east/north leave, west/south enter
+    for (int q = 0; q < 4; ++q) { L[q] += sg * a * F[q]; LY[q] += sg * a * FY[q]; }
+    LO += sg * a * FO;
+  }
+}
+
+// T1/T2 of an interior cell from global memory
+template <typename R>
+__device__ void vcell_global(const DevInst& I, const R* U, R t, R dt, int i, int j, R* T1, R* T2) {
+  const Prim<R> w0 = prim_at<R>(I, U, i, j, t);
+  for (int q = 0; q < 4; ++q) { T1[q] = R(0); T2[q] = R(0); }
+  if (w0.r < R(0)) return;
+  const Prim<R> wE = prim_at<R>(I, U, i + 1, j, t), wW = prim_at<R>(I, U, i - 1, j, t);
+  const Prim<R> wN = prim_at<R>(I, U, i, j + 1, t), wS = prim_at<R>(I, U, i, j - 1, t);
+  vcell<R>(w0, wE, wW, wN, wS, i, j, R(I.lt[(long long)j * I.Nx + i]), dt, I.c, T1, T2);
+}
+
+// a cell's own net viscous flux from global memory
+template <typename R>
+__device__ void own_vflux_global(const DevInst& I, const R* U, R t, R dt, int ci, int cj, R* L, R* LY) {
+  const int Nx = I.Nx, Nr = I.Nr;
+  for (int f = 0; f < 4; ++f) {
+    const bool dirR = f >= 2;
+    const int fi = ci + (f == 1 ? 1 : 0), fj = cj + (f == 3 ? 1 : 0);
+    const float area = dirR ? I.ar[(long long)fj * Nx + fi] : I.ax[(long long)fj * (Nx + 1) + fi];
+    if (area == 0.f) continue;
+    if (dirR ? (fj == 0 || fj >= Nr) : (fi == 0 || fi >= Nx)) continue;
+    const int k1i = dirR ? fi : fi - 1, k1j = dirR ? fj - 1 : fj;
+    const Prim<R> w1 = prim_at<R>(I, U, k1i, k1j, t), w2 = prim_at<R>(I, U, fi, fj, t);
+    if (w1.r < R(0) || w2.r < R(0)) continue;
+    R t1[4], m1[4], t2[4], m2[4], F[4], FY[3];
+    vcell_global<R>(I, U, t, dt, k1i, k1j, t1, m1);
+    vcell_global<R>(I, U, t, dt, fi, fj, t2, m2);
+    visc_face<R>(dirR, fj, w1, w2, t1, t2, m1, m2, F, FY);
+    const R a = R(fabsf(area));
+    const R sg = (f == 1 || f == 3) ? R(-1) : R(1);
+    for (int q = 0; q < 4; ++q) L[q] += sg * a * F[q];
+    for (int q = 0; q < 3; ++q) LY[q] += sg * a * FY[q];
+  }
+}
+
+// ------------------------------------------------------------------------------
+template <typename R, int TX, int TY>
+struct StageSmem {
+  static constexpr int PW = TX + 4, PH = TY + 4;   // primitive tile with 2-cell halo
+  static constexpr int NFX = (TX + 1) * TY, NFR = TX * (TY + 1);
+  static constexpr int TW = TX + 2, THh = TY + 2;  // T1/T2 tile with 1-cell ring
+  R P[8][PH][PW];
+  union {
+    struct {
+      R Fx[9][TY][TX + 1];
+      R Fr[9][TY + 1][TX];
+    } inv;
+    struct {
+      R T[8][THh][TW];
+      R Vx[7][TY][TX + 1];
+      R Vr[7][TY + 1][TX];
+    } vis;
+  } u;
+  float red[16];
+  double redd[16];
+  int last;
+};
+
+template <typename R, int TX, int TY, int STAGE, bool PHYS>
+__global__ void __launch_bounds__(TX* TY) k_stage(const DevInst* __restrict__ insts, int ninst, unsigned* __restrict__ done) {
+  constexpr int NT = TX * TY;
+  using S = StageSmem<R, TX, TY>;
+  __shared__ S sm;
+  const int tid = threadIdx.x;
+  const int t = blockIdx.x;
+  const int ii = find_inst(insts, ninst, t);
+  const DevInst& I = insts[ii];
+  const Consts& c = I.c;
+  InstState* st = I.st;
+  const bool active = !(st->flags & 1u);
+  const int Nx = I.Nx, Nr = I.Nr;
+  const long long Ns = I.Ns;
+  const int lt_ = t - I.tile0;
+  const int i0 = (lt_ % I.tilesX) * TX, j0 = (lt_ / I.tilesX) * TY;
+  R* __restrict__ U0 = static_cast<R*>(I.U0);
+  R* __restrict__ U1 = static_cast<R*>(I.U1);
+  const R* __restrict__ Uin = STAGE == 1 ? U0 : U1;
+
+  // time step of this step: FS_STEPINFO at the start of the step
+  float cur[4];
+  if (STAGE == 1) {
+    step_info(c, *st, cur);
+    if (lt_ == 0 && tid == 0 && active) { st->cur[0] = cur[0]; st->cur[1] = cur[1]; st->cur[2] = cur[2]; st->cur[3] = cur[3]; }
+  } else {
+    cur[0] = st->cur[0]; cur[1] = st->cur[1]; cur[2] = st->cur[2]; cur[3] = st->cur[3];
+  }
+  const R dt = R(cur[0]);
+  const R tsel = R(STAGE == 1 ? cur[1] : cur[2]);
+
+  float wloc = 0.f;
+  double rloc = 0.0;
+  if (active) {
+    // ---- 1. primitive tile with ghost halo (FS_PRIM)
+    for (int idx = tid; idx < S::PW * S::PH; idx += NT) {
+      const int li = idx % S::PW, lj = idx / S::PW;
+      const Prim<R> q = prim_at<R>(I, Uin, i0 - 2 + li, j0 - 2 + lj, tsel);
+      sm.P[0][lj][li] = q.r; sm.P[1][lj][li] = q.u; sm.P[2][lj][li] = q.v; sm.P[3][lj][li] = q.p;
+      sm.P[4][lj][li] = q.Z; sm.P[5][lj][li] = q.F; sm.P[6][lj][li] = q.G; sm.P[7][lj][li] = q.O;
+    }
+    __syncthreads();
+    auto P_at = [&](int li, int lj) {
+      Prim<R> q;
+      q.r = sm.P[0][lj][li]; q.u = sm.P[1][lj][li]; q.v = sm.P[2][lj][li]; q.p = sm.P[3][lj][li];
+      q.Z = sm.P[4][lj][li]; q.F = sm.P[5][lj][li]; q.G = sm.P[6][lj][li]; q.O = sm.P[7][lj][li];
+      return q;
+    };
+
+    // ---- 2. each face flux once (FS_FLUX_X / FS_FLUX_R), times |area|
+    for (int f = tid; f < S::NFX + S::NFR; f += NT) {
+      const bool dirR = f >= S::NFX;
+      const int g = dirR ? f - S::NFX : f;
+      const int fx = dirR ? g % TX : g % (TX + 1);
+      const int fy = dirR ? g / TX : g / (TX + 1);
+      const int gi = i0 + fx, gj = j0 + fy;
+      R F[4] = {0, 0, 0, 0}, FY[4] = {0, 0, 0, 0}, FO = 0;
+      float area = 0.f;
+      const bool inDom = dirR ? (gi < Nx && gj <= Nr) : (gi <= Nx && gj < Nr);
+      if (inDom) area = dirR ? I.ar[(long long)gj * Nx + gi] : I.ax[(long long)gj * (Nx + 1) + gi];
+      if (area != 0.f && !(dirR && gj == 0)) {
+        const int dx = dirR ? 0 : 1, dy = dirR ? 1 : 0;
+        const int li = fx + 2, lj = fy + 2;
+        Prim<R> c0 = P_at(li - 2 * dx, lj - 2 * dy), c1 = P_at(li - dx, lj - dy);
+        Prim<R> c2 = P_at(li, lj), c3 = P_at(li + dx, lj + dy);
+        const bool s1 = c1.r < R(0), s2 = c2.r < R(0);
+        if (!(s1 && s2)) {
+          R hllw = R(0);
+          if (c.hllBlend > 0.f && !(s1 || s2)) {
+            const int tx = dy, ty = dx;
+            const R pa1 = sm.P[0][lj - dy + ty][li - dx + tx] < R(0) ? c1.p : sm.P[3][lj - dy + ty][li - dx + tx];
+            const R pb1 = sm.P[0][lj - dy - ty][li - dx - tx] < R(0) ? c1.p : sm.P[3][lj - dy - ty][li - dx - tx];
+            const R pa2 = sm.P[0][lj + ty][li + tx] < R(0) ? c2.p : sm.P[3][lj + ty][li + tx];
+            const R pb2 = sm.P[0][lj - ty][li - tx] < R(0) ? c2.p : sm.P[3][lj - ty][li - tx];
+            hllw = hll_weight<R>(c1.p, c2.p, pa1, pb1, pa2, pb2, R(c.hllBlend));
+          }
+          if (dirR) {
+            R s;
+            s = c0.u; c0.u = c0.v; c0.v = s;
+            s = c1.u; c1.u = c1.v; c1.v = s;
+            s = c2.u; c2.u = c2.v; c2.v = s;
+            s = c3.u; c3.u = c3.v; c3.v = s;
+          }
+          face_flux<R>(c0, c1, c2, c3, hllw, c.order2 && area > 0.f, F, FY, FO);
+          if (dirR) { const R s = F[1]; F[1] = F[2]; F[2] = s; }
+          const R a = R(fabsf(area));
+          for (int q = 0; q < 4; ++q) { F[q] *= a; FY[q] *= a; }
+          FO *= a;
+        }
+      }
+      if (dirR) {
+        for (int q = 0; q < 4; ++q) { sm.u.inv.Fr[q][fy][fx] = F[q]; sm.u.inv.Fr[4 + q][fy][fx] = FY[q]; }
+        sm.u.inv.Fr[8][fy][fx] = FO;
+      } else {
+        for (int q = 0; q < 4; ++q) { sm.u.inv.Fx[q][fy][fx] = F[q]; sm.u.inv.Fx[4 + q][fy][fx] = FY[q]; }
+        sm.u.inv.Fx[8][fy][fx] = FO;
+      }
+    }
+    __syncthreads();
+
+    // ---- 3. merged-group finite-volume update (FS_UPDATE)
+    const int tx = tid % TX, ty = tid / TX;
+    const int ci = i0 + tx, cj = j0 + ty;
+    const bool inDom = ci < Nx && cj < Nr;
+    const long long k = (long long)cj * Nx + ci;
+    const uint32_t code = inDom ? I.code[k] : C_IS_SLAVE;
+    const bool solid = (code & C_SOLID) != 0;
+    const bool host = inDom && !solid && !(code & C_IS_SLAVE);
+    R A[4], B[4];         // group state after the stage
+    R ubarM[5];           // member-specific B.w (inflow-speed average in column 0)
+    int mi[5], mj[5], nm = 0;
+    bool guard = false;   // NaN guard hit
+    R rho_old = R(0);
+    float4 gg = make_float4(0.f, 0.f, 0.f, 0.f);
+    Prim<R> qS;           // state after the stage, as primitives (physics input)
+    if (host) {
+      mi[0] = ci; mj[0] = cj; nm = 1;
+      const uint32_t sl = code_slaves(code);
+      if (sl & 1u) { mi[nm] = ci - 1; mj[nm] = cj; ++nm; }
+      if (sl & 2u) { mi[nm] = ci + 1; mj[nm] = cj; ++nm; }
+      if (sl & 4u) { mi[nm] = ci; mj[nm] = cj - 1; ++nm; }
+      if (sl & 8u) { mi[nm] = ci; mj[nm] = cj + 1; ++nm; }
+      R L[4], LY[4], LO;
+      for (int q = 0; q < 4; ++q) {
+        L[q] = -(sm.u.inv.Fx[q][ty][tx + 1] - sm.u.inv.Fx[q][ty][tx]) - (sm.u.inv.Fr[q][ty + 1][tx] - sm.u.inv.Fr[q][ty][tx]);
+        LY[q] = -(sm.u.inv.Fx[4 + q][ty][tx + 1] - sm.u.inv.Fx[4 + q][ty][tx]) - (sm.u.inv.Fr[4 + q][ty + 1][tx] - sm.u.inv.Fr[4 + q][ty][tx]);
+      }
+      LO = -(sm.u.inv.Fx[8][ty][tx + 1] - sm.u.inv.Fx[8][ty][tx]) - (sm.u.inv.Fr[8][ty + 1][tx] - sm.u.inv.Fr[8][ty][tx]);
+      for (int m = 1; m < nm; ++m) {
+        const int sx = mi[m] - i0, sy = mj[m] - j0;
+        if (sx >= 0 && sx < TX && sy >= 0 && sy < TY) {
+          for (int q = 0; q < 4; ++q) {
+            L[q] += -(sm.u.inv.Fx[q][sy][sx + 1] - sm.u.inv.Fx[q][sy][sx]) - (sm.u.inv.Fr[q][sy + 1][sx] - sm.u.inv.Fr[q][sy][sx]);
+            LY[q] += -(sm.u.inv.Fx[4 + q][sy][sx + 1] - sm.u.inv.Fx[4 + q][sy][sx]) - (sm.u.inv.Fr[4 + q][sy + 1][sx] - sm.u.inv.Fr[4 + q][sy][sx]);
+          }
+          LO += -(sm.u.inv.Fx[8][sy][sx + 1] - sm.u.inv.Fx[8][sy][sx]) - (sm.u.inv.Fr[8][sy + 1][sx] - sm.u.inv.Fr[8][sy][sx]);
+        } else {
+          own_flux_global<R>(I, Uin, tsel, mi[m], mj[m], L, LY, LO);
+        }
+      }
+      gg = I.gg[k];
+      R a[4], b[4];
+      for (int q = 0; q < 4; ++q) { a[q] = Uin[q * Ns + k]; b[q] = Uin[(4 + q) * Ns + k]; }
+      const Prim<R> q = to_prim<R>(a[0], a[1], a[2], a[3], b[0], b[1], b[2], b[3]);
+      const R Wx = R(gg.z), Wr = R(gg.w);
+      const R wl = sqrt(Wx * Wx + Wr * Wr);
+      if (wl > R(1e-6)) {
+        const R pw = wall_pressure<R>(q, q.u * (Wx / wl) + q.v * (Wr / wl));
+        L[1] -= Wx * pw;
+        L[2] -= Wr * pw;
+      }
+      L[2] += q.p * R(gg.y);
+      const R invV = R(1) / fmax(R(gg.x), R(1e-6));
+      const R LB[4] = {LY[0], LY[1], LY[2] - q.G * LY[3], LO};
+      R Uq[4], UY[4];
+      if (STAGE == 1) {
+        for (int m = 0; m < 4; ++m) { Uq[m] = a[m] + dt * invV * L[m]; UY[m] = b[m] + dt * invV * LB[m]; }
+      } else {
+        R a0[4], b0[4];
+        for (int m = 0; m < 4; ++m) { a0[m] = U0[m * Ns + k]; b0[m] = U0[(4 + m) * Ns + k]; }
+        rho_old = a0[0];
+        for (int m = 0; m < 4; ++m) {
+          Uq[m] = R(0.5) * (a0[m] + a[m] + dt * invV * L[m]);
+          UY[m] = R(0.5) * (b0[m] + b[m] + dt * invV * LB[m]);
+        }
+      }
+      const R rho = fmax(Uq[0], R(1e-7));
+      R vx = Uq[1] / fmax(Uq[0], R(1e-7)), vy = Uq[2] / fmax(Uq[0], R(1e-7));
+      const R sp = sqrt(vx * vx + vy * vy);
+      if (sp > R(c.vMax)) { vx *= R(c.vMax) / sp; vy *= R(c.vMax) / sp; }
+      if (!(rho < R(1e30)) || !(vx < R(1e30) && vx > R(-1e30))) {
+        guard = true;
+        for (int m = 0; m < 4; ++m) { A[m] = R(c.ambCons[m]); B[m] = R(c.ambB[m]); }
+        for (int m = 0; m < nm; ++m) ubarM[m] = R(c.ambB[3]);
+      } else {
+        const R ke = R(0.5) * rho * (vx * vx + vy * vy);
+        const R Gn = clampR(UY[2], R(c.gamR[0]), R(c.gamR[1]));
+        const R E = fmax(Uq[3], R(1e-7) * Gn + ke);
+        const R Zn = clampR(UY[0], R(0), rho);
+        A[0] = rho; A[1] = rho * vx; A[2] = rho * vy; A[3] = E;
+        B[0] = Zn; B[1] = clampR(UY[1], R(0), Zn); B[2] = Gn;
+        B[3] = c.inj ? clampR(UY[3], R(0), rho) : R(0);
+        for (int m = 0; m < nm; ++m) {
+          R ub = B[3];
+          if (!c.inj) {
+            ub = R(0);
+            if (mi[m] == 0) {
+              const R u0 = Uin[7 * Ns + (long long)mj[m] * Nx];
+              const bool on = STAGE == 2 && R(cur[2]) > R(c.uBar[2]);
+              ub = on ? fmin(u0 + fmin(dt * R(c.uBar[0]), R(1)) * (vx - u0), R(c.uBar[1])) : u0;
+            }
+          }
+          ubarM[m] = ub;
+        }
+      }
+      qS = to_prim<R>(A[0], A[1], A[2], A[3], B[0], B[1], B[2], ubarM[0]);
+    }
+
+    // ---- 4. viscous / wall / afterburning split step (FS_VCELL, FS_VFLUX, FS_SOURCE)
+    if (STAGE == 2 && PHYS) {
+      __syncthreads();  // inviscid fluxes consumed; reuse their LDS
+      for (int idx = tid; idx < S::TW * S::THh; idx += NT) {
+        const int ri = idx % S::TW, rj = idx / S::TW;
+        const int li = ri + 1, lj = rj + 1;
+        const int gi = i0 - 2 + li, gj = j0 - 2 + lj;
+        R T1[4] = {0, 0, 0, 0}, T2[4] = {0, 0, 0, 0};
+        if (gi >= 0 && gi < Nx && gj >= 0 && gj < Nr && sm.P[0][lj][li] >= R(0)) {
+          vcell<R>(P_at(li, lj), P_at(li + 1, lj), P_at(li - 1, lj), P_at(li, lj + 1), P_at(li, lj - 1), gi, gj,
+                   R(I.lt[(long long)gj * Nx + gi]), dt, c, T1, T2);
+        }
+        for (int q = 0; q < 4; ++q) { sm.u.vis.T[q][rj][ri] = T1[q]; sm.u.vis.T[4 + q][rj][ri] = T2[q]; }
+      }
+      __syncthreads();
+      if (c.visc) {
+        for (int f = tid; f < S::NFX + S::NFR; f += NT) {
+          const bool dirR = f >= S::NFX;
+          const int g = dirR ? f - S::NFX : f;
+          const int fx = dirR ? g % TX : g % (TX + 1);
+          const int fy = dirR ? g / TX : g / (TX + 1);
+          const int gi = i0 + fx, gj = j0 + fy;
+          R F[4] = {0, 0, 0, 0}, FY[3] = {0, 0, 0};
+          bool on = dirR ? (gi < Nx && gj > 0 && gj < Nr) : (gi > 0 && gi < Nx && gj < Nr);
+          float area = 0.f;
+          if (on) area = dirR ? I.ar[(long long)gj * Nx + gi] : I.ax[(long long)gj * (Nx + 1) + gi];
+          if (on && area != 0.f) {
+            const int li = fx + 2, lj = fy + 2;
+            const int l1i = dirR ? li : li - 1, l1j = dirR ? lj - 1 : lj;
+            const Prim<R> w1 = P_at(l1i, l1j), w2 = P_at(li, lj);
+            if (!(w1.r < R(0) || w2.r < R(0))) {
+              R t1[4], m1[4], t2[4], m2[4];
+              for (int q = 0; q < 4; ++q) {
+                t1[q] = sm.u.vis.T[q][l1j - 1][l1i - 1]; m1[q] = sm.u.vis.T[4 + q][l1j - 1][l1i - 1];
+                t2[q] = sm.u.vis.T[q][lj - 1][li - 1]; m2[q] = sm.u.vis.T[4 + q][lj - 1][li - 1];
+              }
+              visc_face<R>(dirR, gj, w1, w2, t1, t2, m1, m2, F, FY);
+              const R a = R(fabsf(area));
+              for (int q = 0; q < 4; ++q) F[q] *= a;
+              for (int q = 0; q < 3; ++q) FY[q] *= a;
+            }
+          }
+          if (dirR) {
+            for (int q = 0; q < 4; ++q) sm.u.vis.Vr[q][fy][fx] = F[q];
+            for (int q = 0; q < 3; ++q) sm.u.vis.Vr[4 + q][fy][fx] = FY[q];
+          } else {
+            for (int q = 0; q < 4; ++q) sm.u.vis.Vx[q][fy][fx] = F[q];
+            for (int q = 0; q < 3; ++q) sm.u.vis.Vx[4 + q][fy][fx] = FY[q];
+          }
+        }
+        __syncthreads();
+      }
+      if (host) {
+        const R rho = qS.r, p = qS.p, G = qS.G, velx = qS.u, vely = qS.v;
+        const R V = fmax(R(gg.x), R(1e-6));
+        const R Rm = rmix(qS.Z, qS.F, qS.O, c), Tg = p / (rho * Rm);
+        R L[4] = {0, 0, 0, 0}, LY[3] = {0, 0, 0};
+        R T1h[4], T2h[4];
+        for (int q = 0; q < 4; ++q) { T1h[q] = sm.u.vis.T[q][ty + 1][tx + 1]; T2h[q] = sm.u.vis.T[4 + q][ty + 1][tx + 1]; }
+        if (c.visc) {
+          for (int m = 0; m < nm; ++m) {
+            const int sx = mi[m] - i0, sy = mj[m] - j0;
+            if (sx >= 0 && sx < TX && sy >= 0 && sy < TY) {
+              for (int q = 0; q < 4; ++q)
+                L[q] += -(sm.u.vis.Vx[q][sy][sx + 1] - sm.u.vis.Vx[q][sy][sx]) - (sm.u.vis.Vr[q][sy + 1][sx] - sm.u.vis.Vr[q][sy][sx]);
+              for (int q = 0; q < 3; ++q)
+                LY[q] += -(sm.u.vis.Vx[4 + q][sy][sx + 1] - sm.u.vis.Vx[4 + q][sy][sx]) - (sm.u.vis.Vr[4 + q][sy + 1][sx] - sm.u.vis.Vr[4 + q][sy][sx]);
+            } else {
+              own_vflux_global<R>(I, U1, tsel, dt, mi[m], mj[m], L, LY);
+            }
+          }
+          // axisymmetric hoop stress (solver.js:587-588)
+          const R r = V / fmax(R(gg.y), R(1e-6)), vor = vely / r;
+          L[2] -= T2h[0] * (R(2) * vor - R(0.6666667f) * (T1h[0] + T1h[3] + vor)) * R(gg.y);
+          // Werner-Wengle wall shear + Reynolds-analogy heat flux (solver.js:590-607)
+          const R gx = R(gg.z), gr = R(gg.w);
+          const R wl = sqrt(gx * gx + gr * gr);
+          if (wl > R(1e-6)) {
+            const R nx = gx / wl, nr = gr / wl;
+            const R vn = velx * nx + vely * nr;
+            const R utx = velx - vn * nx, utr = vely - vn * nr;
+            const R up = sqrt(utx * utx + utr * utr);
+            const R cp = Rm * (G + R(1));
+            const R Taw = Tg + R(0.89) * up * up / (R(2) * cp);
+            const R Ts = fmax(Tg + R(0.5) * (R(c.Tw) - Tg) + R(0.22) * (Taw - Tg), R(1e-3));
+            const R rs = p / (Rm * Ts), mus = sutherland(Ts * R(c.gas[2]), c);
+            const R dz = R(2) * clampR(R(0.5) * V / wl, R(0.25), R(1));
+            const R nzv = mus / (rs * dz);
+            R tw = up <= R(69.7) * nzv ? R(2) * mus * up / dz
+                                       : rs * pow(R(7.187) * pow(nzv, R(1.142857)) + R(0.13769) * pow(nzv, R(0.142857)) * up, R(1.75));
+            tw = fmin(tw, R(0.3) * rho * up * V / fmax(wl * dt, R(1e-9)));
+            if (up > R(1e-6)) { L[1] -= tw * wl * utx / up; L[2] -= tw * wl * utr / up; }
+            const R qw = tw / fmax(up, R(1e-4)) * cp * (Taw - R(c.Tw)) * R(1.24);
+            const R qmax = R(0.3) * p * G * V / fmax(wl * dt, R(1e-9));
+            L[3] -= clampR(qw, -qmax, qmax) * wl;
+          }
+          // Zukoski separation in the boundary-layer band (solver.js:613-630)
+          if (code & C_BL) {
+            const R up = sqrt(velx * velx + vely * vely), Mn = up / sqrt((R(1) + R(1) / G) * p / rho);
+            if (Mn > R(1)) {
+              const R shx = velx / up, shy = vely / up;
+              R pd = R(0);
+              for (int n = 4; n <= 6; n += 2)
+                for (int m = 0; m <= 1; ++m) {
+                  int Dx = ci + (int)round(R(n) * shx);
+                  int Dy = cj + (int)floor(R(n) * shy) + m;
+                  Dx = min(max(Dx, 0), Nx - 1);
+                  Dy = min(max(Dy, 0), Nr - 1);
+                  const long long kd = (long long)Dy * Nx + Dx;
+                  if (!(I.code[kd] & C_SOLID)) pd = fmax(pd, load_prim<R>(U1, Ns, kd).p);
+                }
+              const R crit = R(1) + R(0.5) * Mn;
+              const R fs = smoothstepR(crit, R(1.2) * crit, pd / p) * V * rho * up;
+              L[1] -= fs * velx;
+              L[2] -= fs * vely;
+            } else if (velx > R(0) && p < R(1.3) * R(c.chemK[3])) {
+              L[1] -= R(0.5) * V * rho * up * velx;
+              L[2] -= R(0.5) * V * rho * up * vely;
+            }
+          }
+        }
+        const R invV = dt / V;
+        const R dO = invV * L[0];
+        R Uq[4] = {A[0], A[1] + invV * L[1], A[2] + invV * L[2], A[3] + invV * L[3]};
+        R UY[4] = {B[0] + invV * LY[0], B[1] + invV * LY[1], B[2] + invV * LY[2] / rho, B[3]};
+        if (c.inj) UY[3] += dO;
+        // afterburning / injector combustion (solver.js:639-654)
+        if (c.chem) {
+          const R Z = qS.Z, Fv = qS.F, O = qS.O, s = R(c.chemK[1]);
+          const R avail = c.inj ? fmin(Fv, (O + R(0.233) * fmax(R(1) - Z, R(0))) / s)
+                                : fmin(Fv, fmax(R(1) - Z - s * (Z - Fv), R(0)) / s);
+          if (avail > R(1e-5)) {
+            const R TK = Tg * R(c.gas[2]);
+            const R kin = R(c.chemK[2]) * p * exp(R(-9000) / TK) * (R(1) - smoothstepR(R(c.cap[0]), R(c.cap[1]), TK));
+            const R mixr = c.inj ? fmax(T2h[2], R(4) * T2h[0] / rho) : T2h[2];
+            const R burn = avail * fmin(fmin(mixr, kin) * dt, R(0.5));
+            UY[1] -= rho * burn;
+            if (c.inj) UY[3] = fmax(UY[3] - rho * burn * s, R(0));
+            Uq[3] += rho * burn * R(c.chemK[0]);
+          }
+        }
+        const R rn = fmax(Uq[0], R(1e-7));
+        const R vnx = Uq[1] / rn, vny = Uq[2] / rn;
+        if (!(rn < R(1e30)) || !(vnx < R(1e30) && vnx > R(-1e30))) {
+          guard = true;
+          for (int m = 0; m < 4; ++m) { A[m] = R(c.ambCons[m]); B[m] = R(c.ambB[m]); }
+          for (int m = 0; m < nm; ++m) ubarM[m] = R(c.ambB[3]);
+        } else {
+          const R Gn = clampR(UY[2], R(c.gamR[0]), R(c.gamR[1]));
+          const R Zn = clampR(UY[0], R(0), rn);
+          A[0] = rn; A[1] = rn * vnx; A[2] = rn * vny;
+          A[3] = fmax(Uq[3], R(1e-7) * Gn + R(0.5) * rn * (vnx * vnx + vny * vny));
+          B[0] = Zn; B[1] = clampR(UY[1], R(0), Zn); B[2] = Gn;
+          if (c.inj) {
+            B[3] = clampR(UY[3], R(0), rn);
+            for (int m = 0; m < nm; ++m) ubarM[m] = B[3];
+          }
+        }
+      }
+    }
+
+    // ---- 5. spray gas sources (FS_SPRAY_APPLY, spray.js:158-178), consumed and cleared
+    if (STAGE == 2 && c.spray && host) {
+      float* __restrict__ src = I.src;
+      const R s0 = R(src[k]), s1 = R(src[Ns + k]), s2 = R(src[2 * Ns + k]), s3 = R(src[3 * Ns + k]);
+      const R dm = s0 + s1;
+      if (!(dm == R(0) && s2 == R(0) && s3 == R(0))) {
+        src[k] = 0.f; src[Ns + k] = 0.f; src[2 * Ns + k] = 0.f; src[3 * Ns + k] = 0.f;
+        const R invV = R(1) / fmax(R(gg.x), R(1e-6));
+        const R velx = A[1] / fmax(A[0], R(1e-7)), vely = A[2] / fmax(A[0], R(1e-7));
+        const R rn = A[0] + dm * invV;
+        const R mx = A[1] + s2 * invV, my = A[2] + s3 * invV;
+        R E = A[3] + (s0 * R(c.eF) + s1 * R(c.eO) + (velx * s2 + vely * s3)) * invV;
+        const R phi = fmin(dm * invV / rn, R(1));
+        const R Gn = mixR(B[2], R(c.Ge), phi);
+        const R vnx = mx / rn, vny = my / rn;
+        E = fmax(E, R(0.02) * A[3] + R(0.5) * rn * (vnx * vnx + vny * vny) + R(1e-7) * Gn);
+        A[0] = rn; A[1] = mx; A[2] = my; A[3] = E;
+        B[0] = B[0] + dm * invV;
+        B[1] = B[1] + s0 * invV;
+        B[2] = Gn;
+        // every member takes the host's B (spray.js:161-177)
+        const R bw = ubarM[0] + s1 * invV;
+        for (int m = 0; m < nm; ++m) ubarM[m] = bw;
+      }
+    }
+
+    // ---- 6. write the group, wave speed for the next step's CFL, residual
+    R* __restrict__ Uout = STAGE == 1 ? U1 : U0;
+    if (inDom && solid) {
+      for (int q = 0; q < 4; ++q) { Uout[q * Ns + k] = R(c.ambCons[q]); Uout[(4 + q) * Ns + k] = R(c.ambB[q]); }
+    }
+    if (host) {
+      for (int m = 0; m < nm; ++m) {
+        const long long km = (long long)mj[m] * Nx + mi[m];
+        for (int q = 0; q < 4; ++q) Uout[q * Ns + km] = A[q];
+        for (int q = 0; q < 3; ++q) Uout[(4 + q) * Ns + km] = B[q];
+        Uout[7 * Ns + km] = ubarM[m];
+      }
+      if (STAGE == 2) {
+        const Prim<R> qn = to_prim<R>(A[0], A[1], A[2], A[3], B[0], B[1], B[2], ubarM[0]);
+        wloc = (float)(fabs(qn.u) + fabs(qn.v) + R(2) * sound_speed(qn));
+        // slaves are fluid cells holding the same state
+        const double dr = (double)(A[0] - rho_old) / fmax((double)dt, 1e-30);
+        rloc = dr * dr;
+      }
+    }
+  }
+
+  if (STAGE == 2) {
+    const float wb = block_max<NT>(wloc, sm.red);
+    const double rb = block_sum<NT>(rloc, sm.redd);
+    if (tid == 0 && active) {
+      atomicMax(&st->wacc, __float_as_uint(wb));
+      unsafeAtomicAdd(&st->resAcc, rb);
+    }
+    // last block to finish closes the step for every instance in the launch
+    __threadfence();
+    if (tid == 0) sm.last = (atomicAdd(done, 1u) == gridDim.x - 1) ? 1 : 0;
+    __syncthreads();
+    if (sm.last) {
+      __threadfence();
+      for (int n = tid; n < ninst; n += NT) {
+        const DevInst& J = insts[n];
+        InstState* s = J.st;
+        if (s->flags & 1u) continue;
+        s->last[0] = s->cur[0]; s->last[1] = s->cur[1]; s->last[2] = s->cur[2]; s->last[3] = s->cur[3];
+        s->wsnap = __hip_atomic_load(&s->wacc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        s->wacc = 0u;
+        s->steps += 1;
+        const double ra = __hip_atomic_load(&s->resAcc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        s->resLast = sqrt(ra / fmax(s->nGroups, 1.0)) * (double)J.c.resScale;
+        s->resAcc = 0.0;
+        if (J.c.spray) {
+          s->head = (s->head + 4u) % (uint32_t)J.c.NP;
+          s->seed = s->seed + 1u;
+        }
+      }
+      if (tid == 0) *done = 0u;
+    }
+  }
+}
+
+// ------------------------------------------------------------------------------
+// Lagrangian parcels (FS_SPRAY_UPDATE, spray.js:40-121) with the point deposit
+// (VS_SPRAY_DEP, spray.js:131-142) as float atomics into the host cell.
+template <typename R>
+__global__ void __launch_bounds__(256) k_spray(const DevInst* __restrict__ insts, int ninst) {
+  // instance of this block
+  int ii = 0;
+  for (int n = 0; n < ninst; ++n)
+    if ((int)blockIdx.x >= insts[n].pblk0 && (int)blockIdx.x < insts[n].pblk0 + insts[n].npblk) ii = n;
+  const DevInst& I = insts[ii];
+  const Consts& c = I.c;
+  const InstState* st = I.st;
+  if ((st->flags & 1u) || !c.spray) return;
+  const int NP = c.NP;
+  const int idx = ((int)blockIdx.x - I.pblk0) * 256 + threadIdx.x;
+  if (idx >= NP) return;
+  const int Nx = I.Nx, Nr = I.Nr;
+  const long long Ns = I.Ns;
+  const R* __restrict__ U1 = static_cast<const R*>(I.U1);
+  float* __restrict__ pc = I.parc;
+  const float dt = st->cur[0];
+  const uint32_t head = st->head, seed = st->seed;
+  const float ax_ = pc[idx], ay = pc[NP + idx], az = pc[2 * NP + idx], aw = pc[3 * NP + idx];
+  const float bx = pc[4 * NP + idx], by = pc[5 * NP + idx], bz = pc[6 * NP + idx], bw = pc[7 * NP + idx];
+  int rel = idx - (int)head;
+  if (rel < 0) rel += NP;
+  float o[8] = {ax_, ay, az, aw, bx, by, bz, bw};
+  float dmo = 0.f, dpx = 0.f, dpy = 0.f;
+  const int perStep = 4;
+  if (rel < perStep) {
+    // new parcel; the liquid left in the replaced one goes to the gas
+    const uint32_t id = (uint32_t)idx ^ (seed * 2654435761u);
+    const int tt = (int)(rnd(id, 1u) * 255.999f);
+    const float* e1 = I.tab + tt * 4;
+    const float* e2 = I.tab + (256 + tt) * 4;
+    const float* e3 = I.tab + (512 + tt) * 4;
+    const float ang = e1[2] + (rnd(id, 2u) - 0.5f) * 2.0f * e2[2];
+    const float r = fmaxf(e1[1] + (rnd(id, 3u) - 0.5f) * e2[3], 0.05f);
+    const float d = e2[1] * expf((rnd(id, 4u) - 0.5f) * 1.4f);
+    o[0] = e1[0]; o[1] = r; o[2] = e1[3] * cosf(ang); o[3] = e1[3] * sinf(ang);
+    o[4] = d; o[5] = c.sprayMhat * dt / (float)perStep; o[6] = e2[0]; o[7] = e3[0];
+    if (by > 0.f) { dmo = by; dpx = by * az; dpy = by * aw; }
+  } else if (by > 0.f) {
+    const int gi = min(max((int)floorf(ax_), 0), Nx - 1), gj = min(max((int)floorf(ay), 0), Nr - 1);
+    const long long kg = (long long)gj * Nx + gi;
+    if (I.code[kg] & C_SOLID) {
+      // born inside solid hardware: walk out into the flow
+      o[0] = ax_ + 1.0f; o[1] = fmaxf(ay - 0.5f, 0.05f); o[2] = fmaxf(az, 0.f); o[3] = aw;
+    } else {
+      const float* pr = bz < 0.5f ? c.fuelVec : c.oxVec;
+      float vx = az, vy = aw;
+      const float m = by, d = bx, dts = dt * c.sprayTScale;
+      float dn = d, mn = m;
+      const Prim<R> w = load_prim<R>(U1, Ns, kg);
+      {
+        const float wr = (float)w.r, wu = (float)w.u, wv = (float)w.v, wp = (float)w.p;
+        const float rm = (1.0f - (float)w.Z) * c.gas[0] + (float)w.F * c.Rs[0] + (float)w.O * c.Rs[1] +
+                         fmaxf((float)w.Z - (float)w.F - (float)w.O, 0.f) * c.Rs[2];
+        const float T = wp / (wr * rm) * c.gas[2];
+        const float rhoG = wr * 1.225f;
+        const float rvx = wu - vx, rvy = wv - vy;
+        const float Urel = sqrtf(rvx * rvx + rvy * rvy) * 287.6f + 1e-3f;
+        const float mu = 1.458e-6f * T * sqrtf(T) / (T + 110.4f);
+        const float Re = rhoG * Urel * d / mu;
+        const float invTau = 18.0f * mu * (1.0f + 0.15f * powf(fminf(Re, 1000.0f), 0.687f)) / (pr[0] * d * d);
+        const float ex = expf(-dts * invTau);
+        vx = wu + (vx - wu) * ex;
+        vy = wv + (vy - wv) * ex;
+        const float cp = rm * 287.05f * ((float)w.G + 1.0f);
+        const float Bn = fmaxf(cp * (T - pr[1]), 0.f) / pr[2];
+        const float K = 8.0f * (mu / 0.7f) / pr[0] * logf(1.0f + Bn) * (1.0f + 0.27f * sqrtf(Re));
+        const float d2 = d * d - K * dts;
+        dn = d2 > 1e-12f ? sqrtf(d2) : 0.f;
+        const float f = dn / d;
+        mn = dn > 0.f ? m * f * f * f : 0.f;
+        const float we = rhoG * Urel * Urel * dn / pr[3];
+        if (we > 12.0f && dn > 0.f) {
+          const float dst = 12.0f * pr[3] / (rhoG * Urel * Urel);
+          const float tb = 3.0f * dn / Urel * sqrtf(pr[0] / rhoG);
+          dn = mixR(dn, fmaxf(dst, 2e-6f), 1.0f - expf(-dts / tb));
+        }
+      }
+      float xi = bw;
+      if (xi > 0.f && ax_ >= xi) {
+        const uint32_t id = (uint32_t)idx ^ (seed * 2654435761u);
+        const float sp = sqrtf(vx * vx + vy * vy), phi = (rnd(id, 5u) - 0.5f) * 2.0f * c.fan;
+        vx = sp * cosf(phi);
+        vy = sp * sinf(phi);
+        dn *= 0.25f;
+        xi = 0.f;
+      }
+      float x = ax_ + vx * dt, r = ay + vy * dt;
+      if (r < 0.f) { r = -r; vy = -vy; }
+      if (x < 0.f) { x = -x; vx = fabsf(vx) * 0.3f; }
+      auto wallAt = [&](float px, float pr_) {
+        const int qi = (int)floorf(px), qj = (int)floorf(pr_);
+        if (qi < 0 || qj < 0 || qi >= Nx || qj >= Nr) return false;
+        return (I.code[(long long)qj * Nx + qi] & C_SOLID) != 0;
+      };
+      if (wallAt(x, r)) {
+        const bool hx = wallAt(x, ay), hr = wallAt(ax_, r);
+        if (hx) { vx *= -0.3f; x = ax_; }
+        if (hr) { vy *= -0.3f; r = ay; }
+        if (!hx && !hr) { x = ax_; r = ay; vx *= -0.3f; vy *= -0.3f; }
+      }
+      const float dm = m - mn;
+      dmo = dm;
+      dpx = mn * (az - vx) + dm * az;
+      dpy = mn * (aw - vy) + dm * aw;
+      if (x >= (float)Nx || r >= (float)Nr || mn <= 0.f) {
+        o[0] = -1.f; o[1] = -1.f; o[2] = 0.f; o[3] = 0.f; o[4] = 0.f; o[5] = 0.f; o[6] = 0.f; o[7] = 0.f;
+      } else {
+        o[0] = x; o[1] = r; o[2] = vx; o[3] = vy; o[4] = dn; o[5] = mn; o[6] = bz; o[7] = xi;
+      }
+    }
+  }
+  for (int q = 0; q < 8; ++q) pc[q * NP + idx] = o[q];
+  // deposit into the host of the cell the parcel started in
+  if (!(ax_ < 0.f) && !(dmo == 0.f && dpx == 0.f && dpy == 0.f)) {
+    int gi = min(max((int)floorf(ax_), 0), Nx - 1), gj = min(max((int)floorf(ay), 0), Nr - 1);
+    const uint32_t cg = I.code[(long long)gj * Nx + gi];
+    gi += code_hdx(cg);
+    gj += code_hdy(cg);
+    const long long h = (long long)gj * Nx + gi;
+    float* __restrict__ src = I.src;
+    unsafeAtomicAdd(&src[h], dmo * (1.0f - bz));
+    unsafeAtomicAdd(&src[Ns + h], dmo * bz);
+    unsafeAtomicAdd(&src[2 * Ns + h], dpx);
+    unsafeAtomicAdd(&src[3 * Ns + h], dpy);
+  }
+}
+
+// ------------------------------------------------------------------------------
+// wave-speed max of the current state + step-info refresh (after setState / rescale)
+template <typename R>
+__global__ void __launch_bounds__(256) k_cfl_init(const DevInst* __restrict__ insts, int ninst, unsigned* __restrict__ done) {
+  __shared__ float red[16];
+  __shared__ int last;
+  const int tid = threadIdx.x;
+  // grid-stride over the cells of every instance: instance by block ranges
+  int ii = 0;
+  for (int n = 0; n < ninst; ++n)
+    if ((int)blockIdx.x >= insts[n].cblk0 && (int)blockIdx.x < insts[n].cblk0 + insts[n].ncblk) ii = n;
+  const DevInst& I = insts[ii];
+  const long long N = (long long)I.Nx * I.Nr;
+  const R* U = static_cast<const R*>(I.U0);
+  float w = 0.f;
+  for (long long k = (long long)(blockIdx.x - I.cblk0) * 256 + tid; k < N; k += (long long)I.ncblk * 256) {
+    if (I.code[k] & C_SOLID) continue;
+    const Prim<R> q = load_prim<R>(U, I.Ns, k);
+    w = fmaxf(w, (float)(fabs(q.u) + fabs(q.v) + R(2) * sound_speed(q)));
+  }
+  const float wb = block_max<256>(w, red);
+  if (tid == 0) atomicMax(&I.st->wacc, __float_as_uint(wb));
+  __threadfence();
+  if (tid == 0) last = (atomicAdd(done, 1u) == gridDim.x - 1) ? 1 : 0;
+  __syncthreads();
+  if (last) {
+    __threadfence();
+    for (int n = tid; n < ninst; n += 256) {
+      InstState* s = insts[n].st;
+      s->wsnap = __hip_atomic_load(&s->wacc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      s->wacc = 0u;
+    }
+    if (tid == 0) *done = 0u;
+  }
+}
+
+// FS_RESCALE (solver.js:425-436), in place
+template <typename R>
+__global__ void k_rescale(DevInst I, float fp, float fT, float du) {
+  const long long N = (long long)I.Nx * I.Nr;
+  const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= N) return;
+  R* U = static_cast<R*>(I.U0);
+  const long long Ns = I.Ns;
+  const Consts& c = I.c;
+  if (I.code[k] & C_SOLID) {
+    for (int q = 0; q < 4; ++q) { U[q * Ns + k] = R(c.ambCons[q]); U[(4 + q) * Ns + k] = R(c.ambB[q]); }
+    return;
+  }
+  const R bw = U[7 * Ns + k];
+  const Prim<R> q = load_prim<R>(U, Ns, k);
+  const R w = R(1) - smoothstepR(R(0), R(0.03f), q.Z);
+  const R rho = q.r * mixR(R(1), R(fp) / R(fT), w);
+  const R p = q.p * mixR(R(1), R(fp), w);
+  const R vx = q.u + R(du) * w, vy = q.v;
+  U[k] = rho; U[Ns + k] = rho * vx; U[2 * Ns + k] = rho * vy; U[3 * Ns + k] = p * q.G + R(0.5) * rho * (vx * vx + vy * vy);
+  U[4 * Ns + k] = rho * q.Z; U[5 * Ns + k] = rho * q.F; U[6 * Ns + k] = q.G; U[7 * Ns + k] = bw;
+}
+
+// FS_WALLROW (solver.js:668-673)
+template <typename R>
+__global__ void k_wall_row(DevInst I, const float* __restrict__ jw, float* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= I.Nx) return;
+  const int j = min(max((int)jw[i * 4], 0), I.Nr - 1);
+  const Prim<R> q = load_prim<R>(static_cast<const R*>(I.U0), I.Ns, (long long)j * I.Nx + i);
+  out[i * 4] = (float)q.u; out[i * 4 + 1] = (float)q.v; out[i * 4 + 2] = (float)q.p; out[i * 4 + 3] = (float)sound_speed(q);
+}
+
+// T2 of the last physics pass, recomputed from U^(1) (the reference's vis.T2, render.js:426)
+template <typename R>
+__global__ void k_t2(DevInst I, float* __restrict__ out) {
+  const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= (long long)I.Nx * I.Nr) return;
+  const int i = (int)(k % I.Nx), j = (int)(k / I.Nx);
+  const R* U1 = static_cast<const R*>(I.U1);
+  R T1[4], T2[4];
+  vcell_global<R>(I, U1, R(I.st->last[2]), R(I.st->last[0]), i, j, T1, T2);
+  for (int q = 0; q < 4; ++q) out[k * 4 + q] = (float)T2[q];
+}
+
+// interleave a rectangle of planes p0..p0+3 into RGBA rows (the texture layout readPixels returns)
+template <typename R>
+__global__ void k_pack(const R* __restrict__ U, long long Ns, int Nx, int p0, int x, int y, int w, int h, float* __restrict__ out) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= w * h) return;
+  const int i = x + k % w, j = y + k / w;
+  const long long c = (long long)j * Nx + i;
+  for (int q = 0; q < 4; ++q) out[(long long)k * 4 + q] = (float)U[(p0 + q) * Ns + c];
+}
+// de-interleave RGBA A/B into the state planes (setState upload)
+template <typename R>
+__global__ void k_unpack(R* __restrict__ U, long long Ns, long long N, const float* __restrict__ A, const float* __restrict__ B) {
+  const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= N) return;
+  for (int q = 0; q < 4; ++q) { U[q * Ns + k] = R(A[k * 4 + q]); U[(4 + q) * Ns + k] = R(B[k * 4 + q]); }
+}
+// parcels all dead (GPUSpray.reset, spray.js:242-254) and sources cleared
+__global__ void k_spray_reset(float* __restrict__ parc, int NP, float* __restrict__ src, long long Ns) {
+  const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < NP) {
+    parc[k] = -1.f; parc[NP + k] = -1.f;
+    for (int q = 2; q < 8; ++q) parc[(long long)q * NP + k] = 0.f;
+  }
+  if (k < Ns)
+    for (int q = 0; q < 4; ++q) src[q * Ns + k] = 0.f;
+}
+
+// FP64 integrals of the current state: exit plane, throat plane, chamber probe, totals
+template <typename R>
+__global__ void __launch_bounds__(256) k_integrals(DevInst I, int iExit, int jExit, int iThroat, int iChamber, double* __restrict__ out) {
+  __shared__ double red[16];
+  const int tid = threadIdx.x;
+  const R* U = static_cast<const R*>(I.U0);
+  const long long Ns = I.Ns;
+  const int Nx = I.Nx, Nr = I.Nr;
+  const double pa = (double)I.c.amb[3];
+  double F = 0, md = 0, Mw = 0, pw = 0, Aa = 0, mt = 0, mass = 0, en = 0;
+  for (int j = tid; j < jExit; j += 256) {
+    const long long k = (long long)j * Nx + iExit;
+    const double rho = fmax((double)U[k], 1e-9), u = (double)U[Ns + k] / rho, v = (double)U[2 * Ns + k] / rho;
+    const double G = (double)U[6 * Ns + k], g = 1.0 + 1.0 / G;
+    const double p = fmax(((double)U[3 * Ns + k] - 0.5 * rho * (u * u + v * v)) / G, 1e-9);
+    const double dA = 2.0 * M_PI * fabs((double)I.ax[(long long)j * (Nx + 1) + iExit + 1]);
+    const double cs = sqrt(g * p / rho);
+    const double m = (double)U[4 * Ns + k] * u * dA;
+    F += (rho * u * u + p - pa) * dA;
+    md += m;
+    Mw += m * sqrt(u * u + v * v) / cs;
+    pw += p * dA;
+    Aa += dA;
+  }
+  // throat plane: fluid cells from the axis up to the first solid one
+  int jT = 0;
+  while (jT < Nr && !(I.code[(long long)jT * Nx + iThroat] & C_SOLID)) ++jT;
+  for (int j = tid; j < jT; j += 256) {
+    const long long k = (long long)j * Nx + iThroat;
+    const double rho = fmax((double)U[k], 1e-9), u = (double)U[Ns + k] / rho;
+    mt += rho * u * 2.0 * M_PI * fabs((double)I.ax[(long long)j * (Nx + 1) + iThroat]);
+  }
+  for (long long k = tid; k < (long long)Nx * Nr; k += 256) {
+    const uint32_t cd = I.code[k];
+    if ((cd & C_SOLID) || (cd & C_IS_SLAVE)) continue;
+    const double V = (double)I.gg[k].x;
+    mass += (double)U[k] * V;
+    en += (double)U[3 * Ns + k] * V;
+  }
+  double vals[8] = {F, md, Mw, pw, Aa, mt, mass, en};
+  for (int q = 0; q < 8; ++q) {
+    const double s = block_sum<256>(vals[q], red);
+    __syncthreads();
+    if (tid == 0) out[q] = s;
+  }
+  if (tid == 0) {
+    const long long k = iChamber;
+    const double rho = fmax((double)U[k], 1e-9), u = (double)U[Ns + k] / rho, v = (double)U[2 * Ns + k] / rho;
+    const double G = (double)U[6 * Ns + k];
+    const double p = fmax(((double)U[3 * Ns + k] - 0.5 * rho * (u * u + v * v)) / G, 1e-9);
+    const double Y = fmin(fmax((double)U[4 * Ns + k] / rho, 0.0), 1.0);
+    const double Fv = fmax((double)U[5 * Ns + k] / rho, 0.0);
+    const double Ov = I.c.inj ? fmax((double)U[7 * Ns + k] / rho, 0.0) : 0.0;
+    const double Rm = (1 - Y) * 1.0 + Fv * I.c.Rs[0] + Ov * I.c.Rs[1] + fmax(Y - Fv - Ov, 0.0) * I.c.Rs[2];
+    out[8] = p;
+    out[9] = p / (rho * Rm);
+    out[10] = I.st->resLast;
+  }
+}
+
+}  // namespace nzk
+
+// ==============================================================================
+// host-side launchers (the runtime in runtime.hip calls only these)
+#include "nz_launch.h"
+
+namespace nzk {
+
+__global__ void k_set_inst(DevInst* dst, DevInst src) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *dst = src;
+}
+
+void tile_shape(bool fp64, int* tx, int* ty) {
+  *tx = fp64 ? TX64 : TX32;
+  *ty = fp64 ? TY64 : TY32;
+}
+
+hipError_t launch_set_inst(DevInst* dst, const DevInst& src, hipStream_t s) {
+  hipLaunchKernelGGL(k_set_inst, dim3(1), dim3(64), 0, s, dst, src);
+  return hipGetLastError();
+}
+
+template <typename R, int TX, int TY>
+static hipError_t stage_t(int stage, bool phys, const DevInst* d, int ninst, int ntiles, unsigned* done, hipStream_t s) {
+  const dim3 g(ntiles), b(TX * TY);
+  if (stage == 1) hipLaunchKernelGGL((k_stage<R, TX, TY, 1, false>), g, b, 0, s, d, ninst, done);
+  else if (phys) hipLaunchKernelGGL((k_stage<R, TX, TY, 2, true>), g, b, 0, s, d, ninst, done);
+  else hipLaunchKernelGGL((k_stage<R, TX, TY, 2, false>), g, b, 0, s, d, ninst, done);
+  return hipGetLastError();
+}
+hipError_t launch_stage(int stage, bool phys, bool fp64, const DevInst* d, int ninst, int ntiles, unsigned* done, hipStream_t s) {
+  return fp64 ? stage_t<double, TX64, TY64>(stage, phys, d, ninst, ntiles, done, s)
+              : stage_t<float, TX32, TY32>(stage, phys, d, ninst, ntiles, done, s);
+}
+hipError_t launch_spray(bool fp64, const DevInst* d, int ninst, int nblk, hipStream_t s) {
+  if (fp64) hipLaunchKernelGGL((k_spray<double>), dim3(nblk), dim3(256), 0, s, d, ninst);
+  else hipLaunchKernelGGL((k_spray<float>), dim3(nblk), dim3(256), 0, s, d, ninst);
+  return hipGetLastError();
+}
+hipError_t launch_cfl_init(bool fp64, const DevInst* d, int ninst, int nblk, unsigned* done, hipStream_t s) {
+  if (fp64) hipLaunchKernelGGL((k_cfl_init<double>), dim3(nblk), dim3(256), 0, s, d, ninst, done);
+  else hipLaunchKernelGGL((k_cfl_init<float>), dim3(nblk), dim3(256), 0, s, d, ninst, done);
+  return hipGetLastError();
+}
+hipError_t launch_rescale(bool fp64, const DevInst& I, float fp, float fT, float du, hipStream_t s) {
+  const long long N = (long long)I.Nx * I.Nr;
+  const dim3 g((unsigned)((N + 255) / 256)), b(256);
+  if (fp64) hipLaunchKernelGGL((k_rescale<double>), g, b, 0, s, I, fp, fT, du);
+  else hipLaunchKernelGGL((k_rescale<float>), g, b, 0, s, I, fp, fT, du);
+  return hipGetLastError();
+}
+hipError_t launch_wall_row(bool fp64, const DevInst& I, const float* jw, float* out, hipStream_t s) {
+  const dim3 g((I.Nx + 255) / 256), b(256);
+  if (fp64) hipLaunchKernelGGL((k_wall_row<double>), g, b, 0, s, I, jw, out);
+  else hipLaunchKernelGGL((k_wall_row<float>), g, b, 0, s, I, jw, out);
+  return hipGetLastError();
+}
+hipError_t launch_t2(bool fp64, const DevInst& I, float* out, hipStream_t s) {
+  const long long N = (long long)I.Nx * I.Nr;
+  const dim3 g((unsigned)((N + 255) / 256)), b(256);
+  if (fp64) hipLaunchKernelGGL((k_t2<double>), g, b, 0, s, I, out);
+  else hipLaunchKernelGGL((k_t2<float>), g, b, 0, s, I, out);
+  return hipGetLastError();
+}
+hipError_t launch_pack(bool fp64, const void* U, long long Ns, int Nx, int p0, int x, int y, int w, int h, float* out, hipStream_t s) {
+  const dim3 g((unsigned)((w * h + 255) / 256)), b(256);
+  if (fp64) hipLaunchKernelGGL((k_pack<double>), g, b, 0, s, static_cast<const double*>(U), Ns, Nx, p0, x, y, w, h, out);
+  else hipLaunchKernelGGL((k_pack<float>), g, b, 0, s, static_cast<const float*>(U), Ns, Nx, p0, x, y, w, h, out);
+  return hipGetLastError();
+}
+hipError_t launch_unpack(bool fp64, void* U, long long Ns, long long N, const float* A, const float* B, hipStream_t s) {
+  const dim3 g((unsigned)((N + 255) / 256)), b(256);
+  if (fp64) hipLaunchKernelGGL((k_unpack<double>), g, b, 0, s, static_cast<double*>(U), Ns, N, A, B);
+  else hipLaunchKernelGGL((k_unpack<float>), g, b, 0, s, static_cast<float*>(U), Ns, N, A, B);
+  return hipGetLastError();
+}
+hipError_t launch_spray_reset(float* parc, int NP, float* src, long long Ns, hipStream_t s) {
+  const long long n = NP > Ns ? NP : Ns;
+  hipLaunchKernelGGL(k_spray_reset, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, parc, NP, src, Ns);
+  return hipGetLastError();
+}
+hipError_t launch_integrals(bool fp64, const DevInst& I, int iExit, int jExit, int iThroat, int iChamber, double* out, hipStream_t s) {
+  if (fp64) hipLaunchKernelGGL((k_integrals<double>), dim3(1), dim3(256), 0, s, I, iExit, jExit, iThroat, iChamber, out);
+  else hipLaunchKernelGGL((k_integrals<float>), dim3(1), dim3(256), 0, s, I, iExit, jExit, iThroat, iChamber, out);
+  return hipGetLastError();
+}
+
+}  // namespace nzk

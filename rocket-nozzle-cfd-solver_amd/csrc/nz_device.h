@@ -1,0 +1,403 @@
+// nz_device.h — device-side data layout and per-cell / per-face physics of the
+// MI355X nozzle solver. Written for gfx950 (wave64); included by solver.hip only.
+//
+// Layout in HBM (structure of arrays, one instance):
+//   U[8][Ns]      conserved state planes rho, rho*u, rho*v, E, rho*Z, rho*F, G, w
+//                 (w = inflow-speed average ubar, or rho*O in injector mode), row-major j*Nx+i
+//   ax[Nr][Nx+1]  signed r-weighted x-face areas (geoA.z; negative = first-order face)
+//   ar[Nr+1][Nx]  signed r-weighted r-face areas (geoA.w)
+//   gg[Nx*Nr]     float4 group (V, Am, Wx, Wr) (geoG)
+//   code[Nx*Nr]   packed per-cell flags: solid, host offset, slave bits, boundary-layer band
+//   lt[Nx*Nr]     turbulence length scale in cells (geoM.w)
+// The physics restates the reference GLSL (solver.js / spray.js line numbers
+// at each function); the fusion, tiling and data layout are this library's own.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace nzk {
+
+// ---- per-cell code word
+enum : uint32_t {
+  C_SOLID = 1u,
+  C_SLAVE_SHIFT = 1,   // bits 1..4: slave bits 1,2,4,8 of a host (cutcell.js:168-174)
+  C_BL = 1u << 5,      // boundary-layer band flag (geoM.z bit 16, simsetup.js:60)
+  C_IS_SLAVE = 1u << 6,
+  C_HX_SHIFT = 8,      // host offset dx + 1 (2 bits)
+  C_HY_SHIFT = 10,     // host offset dy + 1 (2 bits)
+};
+__device__ __forceinline__ int code_hdx(uint32_t c) { return (int)((c >> C_HX_SHIFT) & 3u) - 1; }
+__device__ __forceinline__ int code_hdy(uint32_t c) { return (int)((c >> C_HY_SHIFT) & 3u) - 1; }
+__device__ __forceinline__ uint32_t code_slaves(uint32_t c) { return (c >> C_SLAVE_SHIFT) & 15u; }
+
+// ---- parameters of one instance (the reference's P block plus derived uniforms)
+struct Consts {
+  float amb[4], res[3], inlet[4], pStart, ramp, RcCells;
+  int inj;
+  int order2;
+  float hllBlend;
+  float ambCons[4], ambB[4], gamR[2];
+  float uBar[3];   // (inlet[0], 1.6*uInlet0, max(ramp,0)) as solver.js:935 passes them
+  float vMax;
+  float cfl, dtMax, wFloor;
+  int visc, chem;
+  float gas[4], Rs[3], chamberNu[2], cap[2], chemK[4], Tw;
+  float sprayMhat, sprayTScale, fuelVec[4], oxVec[4], eF, eO, Ge;
+  float fan;       // impingement fan half-angle (spray.js:271)
+  float resScale;  // residual normalisation 1/rho_chamber
+  int spray;       // spray stepped this step
+  int NP;          // parcels
+};
+
+// per-instance mutable scalars kept on the device between launches
+struct InstState {
+  float cur[4];    // step info of the step in progress (dt, t_old, t_new, w)
+  float last[4];   // step info of the last completed step (readStepInfo)
+  uint32_t wacc;   // max wave speed being accumulated (float bits, >= 0)
+  uint32_t wsnap;  // max wave speed of the current state
+  uint32_t head;   // spray ring head
+  uint32_t seed;   // spray seed
+  long long steps; // completed steps since setState
+  double resAcc;   // sum of squares of d(rho)/dt over groups, this step
+  double resLast;  // RMS of the last completed step
+  double nGroups;  // number of fluid groups (residual normalisation)
+  uint32_t flags;  // bit 0: instance inactive (skipped)
+  uint32_t pad;
+};
+
+struct DevInst {
+  int Nx, Nr;
+  int tilesX, tilesY, tile0;  // tile grid of this instance inside a launch
+  int pblk0, npblk;           // parcel blocks of this instance inside a spray launch
+  int cblk0, ncblk;           // cell blocks of this instance inside a reduction launch
+  long long Ns;               // plane stride in elements
+  void* U0;                   // state U^n (updated in place by stage 2)
+  void* U1;                   // state U^(1)
+  const float* ax;
+  const float* ar;
+  const float4* gg;
+  const uint32_t* code;
+  const float* lt;
+  float* src;                 // 4 planes of Ns: fuel vapour, ox vapour, x momentum, r momentum
+  float* parc;                // 8 planes of NP: x, r, u, v, d, m, type, xi
+  const float* tab;           // spray element table 256 x 3 RGBA
+  InstState* st;
+  Consts c;
+};
+
+template <typename R>
+struct Prim {
+  R r, u, v, p, Z, F, G, O;
+};
+
+template <typename R>
+__device__ __forceinline__ R clampR(R x, R lo, R hi) { return fmin(fmax(x, lo), hi); }
+template <typename R>
+__device__ __forceinline__ R smoothstepR(R e0, R e1, R x) {
+  const R t = clampR<R>((x - e0) / (e1 - e0), R(0), R(1));
+  return t * t * (R(3) - R(2) * t);
+}
+template <typename R>
+__device__ __forceinline__ R mixR(R a, R b, R t) { return a * (R(1) - t) + b * t; }
+
+// toPrim (solver.js:33-39)
+template <typename R>
+__device__ __forceinline__ Prim<R> to_prim(R a0, R a1, R a2, R a3, R b0, R b1, R b2, R b3) {
+  Prim<R> q;
+  q.r = fmax(a0, R(1e-7));
+  q.u = a1 / q.r;
+  q.v = a2 / q.r;
+  q.G = fmax(b2, R(0.5));
+  q.p = fmax((a3 - R(0.5) * q.r * (q.u * q.u + q.v * q.v)) / q.G, R(1e-7));
+  q.Z = clampR(b0 / q.r, R(0), R(1));
+  q.F = clampR(b1 / q.r, R(0), R(1));
+  q.O = clampR(b3 / q.r, R(0), R(1));
+  return q;
+}
+template <typename R>
+__device__ __forceinline__ R sound_speed(const Prim<R>& q) { return sqrt((R(1) + R(1) / q.G) * q.p / q.r); }
+
+template <typename R>
+__device__ __forceinline__ Prim<R> wall_prim() {
+  Prim<R> q;
+  q.r = R(-1); q.u = R(0); q.v = R(0); q.p = R(1);
+  q.Z = R(0); q.F = R(0); q.G = R(0); q.O = R(0);
+  return q;
+}
+
+template <typename R>
+__device__ __forceinline__ Prim<R> load_prim(const R* __restrict__ U, long long Ns, long long k) {
+  return to_prim<R>(U[k], U[Ns + k], U[2 * Ns + k], U[3 * Ns + k], U[4 * Ns + k], U[5 * Ns + k], U[6 * Ns + k], U[7 * Ns + k]);
+}
+
+// Primitive state with boundary ghost cells at cell (i, j), i in [-2, Nx+1],
+// j in [-2, Nr+1]: stagnation inlet with ignition ramp and absorbing face,
+// far field, outflow, axis mirror; solid cells flagged rho = -1 (solver.js:70-125).
+template <typename R>
+__device__ Prim<R> prim_at(const DevInst& I, const R* __restrict__ U, int i, int j, R t) {
+  const Consts& c = I.c;
+  const int Nx = I.Nx, Nr = I.Nr;
+  const long long Ns = I.Ns;
+  const bool mir = j < 0;
+  if (mir) j = -j - 1;
+  Prim<R> q;
+  q.r = R(c.amb[0]); q.u = R(c.amb[1]); q.v = R(c.amb[2]); q.p = R(c.amb[3]);
+  q.Z = R(0); q.F = R(0); q.G = R(c.inlet[3]); q.O = R(0);
+  if (i < 0) {
+    const bool inCh = R(j) + R(0.5) < R(c.RcCells);
+    if (j < Nr && (I.code[(long long)j * Nx] & C_SOLID)) return wall_prim<R>();
+    if (inCh && c.inj) return wall_prim<R>();
+    if (inCh && j < Nr) {
+      const R f = R(c.ramp) > R(0) ? smoothstepR(R(0), R(c.ramp), t) : R(1);
+      const R P0 = mixR(R(c.pStart), R(c.res[0]), f);
+      const R T0 = R(c.res[1]), Rg = R(c.res[2]), G = R(c.inlet[2]), g = R(1) + R(1) / G;
+      const long long k = (long long)j * Nx;
+      const R a0 = U[k], a1 = U[Ns + k];
+      const R ubar = fmax(U[7 * Ns + k], R(0));
+      const R c0 = sqrt(g * Rg * T0);
+      const R ui = clampR(a1 / fmax(a0, R(1e-7)), R(0), R(0.5) * c0);
+      const R ut = ubar * smoothstepR(R(1), R(2), P0 / fmax(R(c.pStart), R(1e-6)));
+      const R u = mixR(ui, ut, R(c.inlet[1]));
+      const R T = T0 - u * u * (g - R(1)) / (R(2) * g * Rg);
+      const R p = P0 * pow(T / T0, g * G);
+      q.r = p / (Rg * T); q.u = u; q.v = R(0); q.p = p;
+      q.Z = R(1); q.F = R(1); q.G = G; q.O = R(0);
+    }
+  } else if (j >= Nr) {
+    const int ci = min(i, Nx - 1), cj = Nr - 1;
+    const long long k = (long long)cj * Nx + ci;
+    if (!(I.code[k] & C_SOLID)) {
+      const Prim<R> tq = load_prim(U, Ns, k);
+      const R cs = sound_speed(tq);
+      if (tq.v >= cs) q = tq;
+      else if (tq.v > R(0)) { q = tq; q.p = R(c.amb[3]); }
+    }
+  } else if (i >= Nx) {
+    const long long k = (long long)j * Nx + (Nx - 1);
+    if (!(I.code[k] & C_SOLID)) {
+      const Prim<R> tq = load_prim(U, Ns, k);
+      const R cs = sound_speed(tq);
+      if (tq.u >= cs) q = tq;
+      else if (tq.u > R(0)) { q = tq; q.p = R(c.amb[3]); }
+      else q.u = fmax(q.u, R(0));
+    }
+  } else {
+    const long long k = (long long)j * Nx + i;
+    if (I.code[k] & C_SOLID) return wall_prim<R>();
+    q = load_prim(U, Ns, k);
+  }
+  if (mir) q.v = -q.v;
+  return q;
+}
+
+// van Leer limited slope of one component (solver.js:156-160)
+template <typename R>
+__device__ __forceinline__ R van_leer(R a, R b) {
+  R den = a + b;
+  if (fabs(den) < R(1e-30)) den = R(1);
+  return R(2) * fmax(a * b, R(0)) / den;
+}
+template <typename R>
+__device__ __forceinline__ R jump_ratio(R a, R b) { return fabs(a - b) / fmin(a, b); }
+
+// Face flux: MUSCL van Leer reconstruction with positivity fallback, wall mirror
+// states, HLLC with Roe-averaged wave speeds blended to HLL by `hllw`
+// (solver.js:177-255). Inputs are in the face-normal frame (u normal).
+// Outputs F = (mass, normal mom, tangential mom, energy), FY = (rhoZ u, rhoF u,
+// uf*Gf, uf), FO = rhoO u.
+template <typename R>
+__device__ __forceinline__ void face_flux(Prim<R> c0, Prim<R> c1, Prim<R> c2, Prim<R> c3, R hllw, bool second,
+                                          R* F, R* FY, R& FO) {
+  const bool s0 = c0.r < R(0), s1 = c1.r < R(0), s2 = c2.r < R(0), s3 = c3.r < R(0);
+  Prim<R> WL, WR;
+  if (s1) { WR = c2; WL = c2; WL.u = -WL.u; }
+  else if (s2) { WL = c1; WR = c1; WR.u = -WR.u; }
+  else {
+    if (s0) { c0 = c1; c0.u = -c0.u; }
+    if (s3) { c3 = c2; c3.u = -c3.u; }
+    WL = c1;
+    WR = c2;
+    if (second) {
+      const R tLr = c1.r + R(0.5) * van_leer(c1.r - c0.r, c2.r - c1.r);
+      const R tLu = c1.u + R(0.5) * van_leer(c1.u - c0.u, c2.u - c1.u);
+      const R tLv = c1.v + R(0.5) * van_leer(c1.v - c0.v, c2.v - c1.v);
+      const R tLp = c1.p + R(0.5) * van_leer(c1.p - c0.p, c2.p - c1.p);
+      const R tRr = c2.r - R(0.5) * van_leer(c2.r - c1.r, c3.r - c2.r);
+      const R tRu = c2.u - R(0.5) * van_leer(c2.u - c1.u, c3.u - c2.u);
+      const R tRv = c2.v - R(0.5) * van_leer(c2.v - c1.v, c3.v - c2.v);
+      const R tRp = c2.p - R(0.5) * van_leer(c2.p - c1.p, c3.p - c2.p);
+      if (tLr > R(0) && tLp > R(0) && tRr > R(0) && tRp > R(0)) {
+        WL.r = tLr; WL.u = tLu; WL.v = tLv; WL.p = tLp;
+        WR.r = tRr; WR.u = tRu; WR.v = tRv; WR.p = tRp;
+        WL.Z = clampR(c1.Z + R(0.5) * van_leer(c1.Z - c0.Z, c2.Z - c1.Z), R(0), R(1));
+        WL.F = clampR(c1.F + R(0.5) * van_leer(c1.F - c0.F, c2.F - c1.F), R(0), R(1));
+        WL.G = clampR(c1.G + R(0.5) * van_leer(c1.G - c0.G, c2.G - c1.G), R(0.5), R(100));
+        WL.O = clampR(c1.O + R(0.5) * van_leer(c1.O - c0.O, c2.O - c1.O), R(0), R(1));
+        WR.Z = clampR(c2.Z - R(0.5) * van_leer(c2.Z - c1.Z, c3.Z - c2.Z), R(0), R(1));
+        WR.F = clampR(c2.F - R(0.5) * van_leer(c2.F - c1.F, c3.F - c2.F), R(0), R(1));
+        WR.G = clampR(c2.G - R(0.5) * van_leer(c2.G - c1.G, c3.G - c2.G), R(0.5), R(100));
+        WR.O = clampR(c2.O - R(0.5) * van_leer(c2.O - c1.O, c3.O - c2.O), R(0), R(1));
+      }
+    }
+  }
+  const R GL = WL.G, GR = WR.G;
+  const R rL = WL.r, uL = WL.u, vL = WL.v, pL = WL.p;
+  const R rR = WR.r, uR = WR.u, vR = WR.v, pR = WR.p;
+  const R cL = sqrt((R(1) + R(1) / GL) * pL / rL), cR = sqrt((R(1) + R(1) / GR) * pR / rR);
+  const R EL = pL * GL + R(0.5) * rL * (uL * uL + vL * vL);
+  const R ER = pR * GR + R(0.5) * rR * (uR * uR + vR * vR);
+  const R sL = sqrt(rL), sR = sqrt(rR);
+  const R isum = R(1) / (sL + sR);
+  const R uT = (sL * uL + sR * uR) * isum;
+  const R HT = (sL * (EL + pL) / rL + sR * (ER + pR) / rR) * isum;
+  const R vT = (sL * vL + sR * vR) * isum;
+  const R GT = (sL * GL + sR * GR) * isum;
+  const R cT = sqrt(fmax((HT - R(0.5) * (uT * uT + vT * vT)) / GT, R(1e-12)));
+  const R SL = fmin(uL - cL, uT - cT);
+  const R SR = fmax(uR + cR, uT + cT);
+  const R mL = rL * uL, mR = rR * uR;
+  const R FL0 = mL, FL1 = mL * uL + pL, FL2 = mL * vL, FL3 = uL * (EL + pL);
+  const R FR0 = mR, FR1 = mR * uR + pR, FR2 = mR * vR, FR3 = uR * (ER + pR);
+  if (SL >= R(0)) {
+    F[0] = FL0; F[1] = FL1; F[2] = FL2; F[3] = FL3;
+    FY[0] = mL * WL.Z; FY[1] = mL * WL.F; FO = mL * WL.O;
+    FY[2] = uL * GL; FY[3] = uL;
+    return;
+  }
+  if (SR <= R(0)) {
+    F[0] = FR0; F[1] = FR1; F[2] = FR2; F[3] = FR3;
+    FY[0] = mR * WR.Z; FY[1] = mR * WR.F; FO = mR * WR.O;
+    FY[2] = uR * GR; FY[3] = uR;
+    return;
+  }
+  const R inv = R(1) / (SR - SL);
+  const R SLR = SL * SR;
+  const R Fh0 = (SR * FL0 - SL * FR0 + SLR * (rR - rL)) * inv;
+  const R Fh1 = (SR * FL1 - SL * FR1 + SLR * (rR * uR - rL * uL)) * inv;
+  const R Fh2 = (SR * FL2 - SL * FR2 + SLR * (rR * vR - rL * vL)) * inv;
+  const R Fh3 = (SR * FL3 - SL * FR3 + SLR * (ER - EL)) * inv;
+  const R Yh0 = (SR * mL * WL.Z - SL * mR * WR.Z + SLR * (rR * WR.Z - rL * WL.Z)) * inv;
+  const R Yh1 = (SR * mL * WL.F - SL * mR * WR.F + SLR * (rR * WR.F - rL * WL.F)) * inv;
+  const R Yh2 = (SR * mL * WL.O - SL * mR * WR.O + SLR * (rR * WR.O - rL * WL.O)) * inv;
+  const R Ss = (pR - pL + mL * (SL - uL) - mR * (SR - uR)) / (rL * (SL - uL) - rR * (SR - uR));
+  const bool left = Ss >= R(0);
+  const R rK = left ? rL : rR, uK = left ? uL : uR, vK = left ? vL : vR, pK = left ? pL : pR;
+  const R EK = left ? EL : ER, SK = left ? SL : SR;
+  const R YK0 = left ? WL.Z : WR.Z, YK1 = left ? WL.F : WR.F, YK2 = left ? WL.O : WR.O;
+  const R FK0 = left ? FL0 : FR0, FK1 = left ? FL1 : FR1, FK2 = left ? FL2 : FR2, FK3 = left ? FL3 : FR3;
+  const R mK = rK * uK;
+  const R kk = rK * (SK - uK) / (SK - Ss);
+  const R Us0 = kk, Us1 = kk * Ss, Us2 = kk * vK, Us3 = kk * (EK / rK + (Ss - uK) * (Ss + pK / (rK * (SK - uK))));
+  F[0] = mixR(FK0 + SK * (Us0 - rK), Fh0, hllw);
+  F[1] = mixR(FK1 + SK * (Us1 - mK), Fh1, hllw);
+  F[2] = mixR(FK2 + SK * (Us2 - rK * vK), Fh2, hllw);
+  F[3] = mixR(FK3 + SK * (Us3 - EK), Fh3, hllw);
+  FY[0] = mixR(mK * YK0 + SK * (kk * YK0 - rK * YK0), Yh0, hllw);
+  FY[1] = mixR(mK * YK1 + SK * (kk * YK1 - rK * YK1), Yh1, hllw);
+  FO = mixR(mK * YK2 + SK * (kk * YK2 - rK * YK2), Yh2, hllw);
+  const R uf = uK + SK * ((SK - uK) / (SK - Ss) - R(1));
+  FY[2] = uf * (left ? GL : GR);
+  FY[3] = uf;
+}
+
+// HLL blending weight from tangential pressure jumps (solver.js:183-190);
+// pX is the pressure of a neighbour or the fall-back when it is a wall.
+template <typename R>
+__device__ __forceinline__ R hll_weight(R p1, R p2, R p1a, R p1b, R p2a, R p2b, R blend) {
+  const R jm = fmax(fmax(jump_ratio(p1a, p1), jump_ratio(p1b, p1)), fmax(jump_ratio(p2a, p2), jump_ratio(p2b, p2)));
+  return smoothstepR(R(0.15), R(0.6), jm * blend);
+}
+
+// reflected-Riemann pressure on a slip wall (solver.js:301-306)
+template <typename R>
+__device__ __forceinline__ R wall_pressure(const Prim<R>& q, R un) {
+  const R g = R(1) + R(1) / q.G, p = q.p, rho = q.r, c = sqrt(g * p / rho);
+  if (un > R(0)) {
+    const R a = R(0.25) * (g + R(1)) * un;
+    return p + rho * un * (a + sqrt(a * a + c * c));
+  }
+  const R f = fmax(R(1) + R(0.5) * (g - R(1)) * un / c, R(0));
+  return p * pow(f, R(2) * g / (g - R(1)));
+}
+
+// Sutherland viscosity and mixture gas constant (solver.js:448-449)
+template <typename R>
+__device__ __forceinline__ R sutherland(R TK, const Consts& c) { return R(c.gas[3]) * R(1.458e-6) * TK * sqrt(TK) / (TK + R(110.4)); }
+template <typename R>
+__device__ __forceinline__ R rmix(R Z, R F, R O, const Consts& c) {
+  return (R(1) - Z) * R(c.gas[0]) + F * R(c.Rs[0]) + O * R(c.Rs[1]) + fmax(Z - F - O, R(0)) * R(c.Rs[2]);
+}
+
+// Per-cell viscous quantities (FS_VCELL, solver.js:458-487): T1 = (ux, ur, vx, vr),
+// T2 = (mu_eff, h, |S|*sensor, mu_lam). Neighbours wE..wS may be walls (r < 0).
+template <typename R>
+__device__ __forceinline__ void vcell(const Prim<R>& w0, const Prim<R>& wE, const Prim<R>& wW, const Prim<R>& wN,
+                                      const Prim<R>& wS, int i, int j, R l, R dt, const Consts& c, R* T1, R* T2) {
+  const R fE = wE.r < R(0) ? R(0) : R(1), fW = wW.r < R(0) ? R(0) : R(1);
+  const R fN = wN.r < R(0) ? R(0) : R(1), fS = wS.r < R(0) ? R(0) : R(1);
+  const R dxn = fmax(fE + fW, R(1)), drn = fmax(fN + fS, R(1));
+  const R ux = (fE * (wE.u - w0.u) + fW * (w0.u - wW.u)) / dxn;
+  const R vx = (fE * (wE.v - w0.v) + fW * (w0.v - wW.v)) / dxn;
+  const R ur = (fN * (wN.u - w0.u) + fS * (w0.u - wS.u)) / drn;
+  const R vr = (fN * (wN.v - w0.v) + fS * (w0.v - wS.v)) / drn;
+  const R r = R(j) + R(0.5), vor = w0.v / r;
+  const R div = ux + vr + vor;
+  const R S2 = R(2) * (ux * ux + vr * vr + vor * vor) + (ur + vx) * (ur + vx) - R(0.6666667f) * div * div;
+  const R om = vx - ur;
+  const R sens = om * om / (om * om + div * div + R(1e-8));
+  const R Smag = sqrt(fmax(S2, R(0))) * sens;
+  const R Rm = rmix(w0.Z, w0.F, w0.O, c);
+  const R mu = sutherland(w0.p / (w0.r * Rm) * R(c.gas[2]), c);
+  R mut = w0.r * l * l * Smag;
+  if (R(i) < R(c.chamberNu[1])) mut = fmax(mut, w0.r * R(c.chamberNu[0]));
+  T1[0] = ux; T1[1] = ur; T1[2] = vx; T1[3] = vr;
+  T2[0] = fmin(mu + mut, R(0.08) * w0.r / fmax(dt, R(1e-9)));
+  T2[1] = (w0.G + R(1)) * w0.p / w0.r;
+  T2[2] = Smag;
+  T2[3] = mu;
+}
+
+// Viscous face flux between cells 1 (west/south) and 2 (FS_VFLUX, solver.js:499-536).
+// F = (ox-vapour diffusion, x mom, r mom, energy), FY = (Z, F, G diffusion).
+template <typename R>
+__device__ __forceinline__ void visc_face(bool dirR, int j, const Prim<R>& w1, const Prim<R>& w2, const R* t1, const R* t2,
+                                          const R* m1, const R* m2, R* F, R* FY) {
+  const R PR_INV = R(1.25);
+  const R mu = R(0.5) * (m1[0] + m2[0]);
+  const R uf = R(0.5) * (w1.u + w2.u), vf = R(0.5) * (w1.v + w2.v);
+  R r, ux, vx, ur, vr;
+  if (dirR) {
+    r = fmax(R(j), R(0.5));
+    ux = R(0.5) * (t1[0] + t2[0]); vx = R(0.5) * (t1[2] + t2[2]);
+    ur = w2.u - w1.u; vr = w2.v - w1.v;
+  } else {
+    r = R(j) + R(0.5);
+    ux = w2.u - w1.u; vx = w2.v - w1.v;
+    ur = R(0.5) * (t1[1] + t2[1]); vr = R(0.5) * (t1[3] + t2[3]);
+  }
+  const R div = ux + vr + vf / r;
+  const R txr = mu * (ur + vx);
+  const R k = mu * PR_INV;
+  F[0] = -k * (w2.O - w1.O);
+  if (dirR) {
+    const R trr = mu * (R(2) * vr - R(0.6666667f) * div);
+    F[1] = -txr; F[2] = -trr; F[3] = -(uf * txr + vf * trr) - k * (m2[1] - m1[1]);
+  } else {
+    const R txx = mu * (R(2) * ux - R(0.6666667f) * div);
+    F[1] = -txx; F[2] = -txr; F[3] = -(uf * txx + vf * txr) - k * (m2[1] - m1[1]);
+  }
+  FY[0] = -k * (w2.Z - w1.Z);
+  FY[1] = -k * (w2.F - w1.F);
+  FY[2] = -k * (w2.G - w1.G);
+}
+
+// PCG hash and uniform in [0,1) (spray.js:33-34)
+__device__ __forceinline__ uint32_t pcg(uint32_t v) {
+  const uint32_t s = v * 747796405u + 2891336453u;
+  const uint32_t w = ((s >> ((s >> 28u) + 4u)) ^ s) * 277803737u;
+  return (w >> 22u) ^ w;
+}
+__device__ __forceinline__ float rnd(uint32_t a, uint32_t n) { return (float)(pcg(a + n * 7919u) & 0xffffffu) / 16777216.0f; }
+
+}  // namespace nzk

@@ -1,0 +1,26 @@
+// nz_launch.h — host-callable launchers of the device kernels (kernels.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "nz_device.h"
+
+namespace nzk {
+
+// tile shapes of the fused stage kernels (cells per workgroup: TX wide in x, TY in r)
+constexpr int TX32 = 64, TY32 = 4;   // FP32: one wave64 per grid row
+constexpr int TX64 = 32, TY64 = 4;   // FP64: half-width tiles keep LDS under 64 KB
+
+void tile_shape(bool fp64, int* tx, int* ty);
+hipError_t launch_set_inst(DevInst* dst, const DevInst& src, hipStream_t s);
+hipError_t launch_stage(int stage, bool phys, bool fp64, const DevInst* d, int ninst, int ntiles, unsigned* done, hipStream_t s);
+hipError_t launch_spray(bool fp64, const DevInst* d, int ninst, int nblk, hipStream_t s);
+hipError_t launch_cfl_init(bool fp64, const DevInst* d, int ninst, int nblk, unsigned* done, hipStream_t s);
+hipError_t launch_rescale(bool fp64, const DevInst& I, float fp, float fT, float du, hipStream_t s);
+hipError_t launch_wall_row(bool fp64, const DevInst& I, const float* jw, float* out, hipStream_t s);
+hipError_t launch_t2(bool fp64, const DevInst& I, float* out, hipStream_t s);
+hipError_t launch_pack(bool fp64, const void* U, long long Ns, int Nx, int p0, int x, int y, int w, int h, float* out, hipStream_t s);
+hipError_t launch_unpack(bool fp64, void* U, long long Ns, long long N, const float* A, const float* B, hipStream_t s);
+hipError_t launch_spray_reset(float* parc, int NP, float* src, long long Ns, hipStream_t s);
+hipError_t launch_integrals(bool fp64, const DevInst& I, int iExit, int jExit, int iThroat, int iChamber, double* out, hipStream_t s);
+
+}  // namespace nzk

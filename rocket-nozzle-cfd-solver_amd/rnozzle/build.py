@@ -1,0 +1,82 @@
+"""In-tree build of the native solver library for gfx950 (hipcc) and of the Node-API addon.
+
+`build()` always recompiles: librnozzle.so (HIP kernels + C ABI + host setup) into
+build_out/, and, when node headers are present, build_out/nozzle_napi.node.
+"""
+import os
+import shutil
+import subprocess
+
+from ._lib import LIB_PATH, ROOT
+
+REPO = os.path.dirname(ROOT)
+CSRC = os.path.join(ROOT, "csrc")
+INCLUDE = os.path.join(REPO, "include")
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = "gfx950"
+
+HIP_SOURCES = ["kernels.hip", "runtime.hip"]
+CPP_SOURCES = ["setup.cpp", "errors.cpp"]
+
+# device code: no packed-FP32 instructions (they are avoided on this platform)
+DEVICE_FLAGS = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+
+
+def _run(cmd, cwd=None):
+    r = subprocess.run(cmd, cwd=cwd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("build failed:\n" + " ".join(cmd) + "\n" + r.stdout[-6000:])
+    return r.stdout
+
+
+def build_lib(jobs=8):
+    out_dir = os.path.dirname(LIB_PATH)
+    obj_dir = os.path.join(out_dir, "obj")
+    os.makedirs(obj_dir, exist_ok=True)
+    common = ["-O3", "-std=c++17", "-fPIC", "-I", INCLUDE, "-I", CSRC]
+    procs = []
+    objs = []
+    for src in HIP_SOURCES:
+        obj = os.path.join(obj_dir, src + ".o")
+        cmd = [HIPCC, f"--offload-arch={ARCH}", "-c", *common, *DEVICE_FLAGS, "-munsafe-fp-atomics",
+               os.path.join(CSRC, src), "-o", obj]
+        procs.append((cmd, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)))
+        objs.append(obj)
+    for src in CPP_SOURCES:
+        obj = os.path.join(obj_dir, src + ".o")
+        cmd = ["g++", "-c", *common, "-D__HIP_PLATFORM_AMD__", os.path.join(CSRC, src), "-o", obj]
+        procs.append((cmd, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)))
+        objs.append(obj)
+    for cmd, p in procs:
+        out, _ = p.communicate()
+        if p.returncode != 0:
+            raise RuntimeError("build failed:\n" + " ".join(cmd) + "\n" + out[-6000:])
+    tmp = LIB_PATH + ".tmp"
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", tmp])
+    os.replace(tmp, LIB_PATH)
+    return LIB_PATH
+
+
+def build_napi():
+    """Node-API addon binding the C ABI for the JavaScript drop-in (js/gpu_solver.js)."""
+    node_inc = "/usr/include/node"
+    if not os.path.exists(os.path.join(node_inc, "node_api.h")):
+        return None
+    out = os.path.join(os.path.dirname(LIB_PATH), "nozzle_napi.node")
+    src = os.path.join(ROOT, "js", "nozzle_napi.cpp")
+    if not os.path.exists(src):
+        return None
+    _run(["g++", "-O2", "-std=c++17", "-shared", "-fPIC", "-I", node_inc, "-I", INCLUDE, src,
+          "-L", os.path.dirname(LIB_PATH), "-lrnozzle", "-Wl,-rpath,$ORIGIN", "-o", out + ".tmp"])
+    os.replace(out + ".tmp", out)
+    return out
+
+
+def build():
+    lib = build_lib()
+    napi = build_napi()
+    return lib, napi
+
+
+def clean():
+    shutil.rmtree(os.path.dirname(LIB_PATH), ignore_errors=True)

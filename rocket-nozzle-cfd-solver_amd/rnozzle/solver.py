@@ -1,0 +1,232 @@
+"""GPUSolver / GPUSpray over the HIP C ABI, with the reference's method names.
+
+Mirrors the JavaScript classes of source/solver.js:677-1076 and
+source/spray.js:211-321 (same methods, arguments, return shapes and error
+behaviour: failures raise, `step` never waits for the device). Arrays are numpy
+float32 in the reference's RGBA texture layout.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import abi
+from ._lib import NozzleError, check, lib
+
+
+def _f(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def _req_floats(reqs):
+    return sum(r["w"] * r["h"] * 4 for r in reqs)
+
+
+class GPUSpray:
+    """GPUSpray (spray.js:211-321): parcels live on the device with the solver."""
+
+    def __init__(self, solver):
+        self.s = solver
+        self.blendOk = True      # float atomics replace EXT_float_blend (spray.js:215)
+        self.ready = True
+        self.n = 65536
+
+    def resize(self):
+        self.reset()
+
+    def setInjector(self, inj):
+        if inj is None:
+            return
+        tab = inj["tab"] if isinstance(inj, dict) else inj
+        t = np.ascontiguousarray(tab, np.float32)
+        if t.size != 256 * 3 * 4:
+            raise NozzleError("setInjector: table must hold 256*3*4 floats")
+        check(lib().nz_spray_set_injector(self.s._h, _f(t)), "setInjector")
+
+    def reset(self):
+        check(lib().nz_spray_reset(self.s._h), "spray.reset")
+
+    def setCount(self, n):
+        check(lib().nz_spray_set_count(self.s._h, int(n)), "spray.setCount")
+        self.n = int(n)
+
+    def stats(self, P=None):
+        st = abi.NzSprayStats()
+        check(lib().nz_spray_stats(self.s._h, C.byref(st)), "spray.stats")
+        return {"fuel": st.fuel, "ox": st.ox, "parcels": int(st.parcels), "meanD": st.meanD}
+
+    def parcels(self):
+        PA = np.zeros(self.n * 4, np.float32)
+        PB = np.zeros(self.n * 4, np.float32)
+        check(lib().nz_spray_read(self.s._h, _f(PA), _f(PB)), "spray.read")
+        return PA, PB
+
+
+class GPUSolver:
+    """GPUSolver (solver.js:677-1076) on one MI355X (HIP stream per solver)."""
+
+    FIELDS = {"A": abi.NZ_FIELD_A, "B": abi.NZ_FIELD_B, "step": abi.NZ_FIELD_STEP,
+              "wallRow": abi.NZ_FIELD_WALLROW, "T2": abi.NZ_FIELD_T2}
+
+    def __init__(self, gl=None, precision=32):
+        h = C.c_void_p()
+        check(lib().nz_create_ex(C.byref(h), int(precision)), "GPUSolver")
+        self._h = h
+        self.precision = precision
+        self.Nx = 0
+        self.Nr = 0
+        self.stepCount = 0
+        self.spray = GPUSpray(self)
+        self._async = None
+        self.mask = None
+
+    # ---- setup (solver.js:759-828)
+    def resize(self, Nx, Nr):
+        check(lib().nz_resize(self._h, int(Nx), int(Nr)), "resize")
+        self.Nx, self.Nr = int(Nx), int(Nr)
+        self._async = None
+
+    def setMask(self, u8):
+        m = np.ascontiguousarray(u8, np.uint8)
+        check(lib().nz_set_mask(self._h, m.ctypes.data_as(C.POINTER(C.c_uint8))), "setMask")
+        self.mask = m
+
+    def setGeometry(self, geo):
+        m = np.ascontiguousarray(geo["mask"], np.uint8)
+        gA = np.ascontiguousarray(geo["geoA"], np.float32)
+        gG = np.ascontiguousarray(geo["geoG"], np.float32)
+        gM = np.ascontiguousarray(geo["geoM"], np.float32)
+        N, NL = self.Nx * self.Nr, (self.Nx + 1) * (self.Nr + 1)
+        if m.size != N or gA.size != NL * 4 or gG.size != N * 4 or gM.size != N * 4:
+            raise NozzleError("setGeometry: array sizes do not match the grid")
+        jw = geo.get("jWall")
+        jwp = None
+        if jw is not None:
+            jw = np.ascontiguousarray(jw, np.float32)
+            jwp = _f(jw)
+        check(lib().nz_set_geometry(self._h, m.ctypes.data_as(C.POINTER(C.c_uint8)), _f(gA), _f(gG), _f(gM), jwp),
+              "setGeometry")
+        self.mask = m
+        self.geoM = gM
+        self.geoG = gG
+
+    def setState(self, A, B):
+        A = np.ascontiguousarray(A, np.float32)
+        B = np.ascontiguousarray(B, np.float32)
+        if A.size != self.Nx * self.Nr * 4 or B.size != A.size:
+            raise NozzleError("setState: arrays must hold Nx*Nr*4 floats")
+        check(lib().nz_set_state(self._h, _f(A), _f(B)), "setState")
+        self.stepCount = 0
+        self._async = None
+
+    # ---- stepping (solver.js:849-891, 998-1010)
+    def step(self, n, P):
+        check(lib().nz_step(self._h, int(n), C.byref(P)), "step")
+        self.stepCount += int(n)
+
+    def rescaleAmbient(self, fp, fT, du, P):
+        check(lib().nz_rescale_ambient(self._h, float(fp), float(fT), float(du), C.byref(P)), "rescaleAmbient")
+
+    def drawWallRow(self):
+        check(lib().nz_draw_wall_row(self._h), "drawWallRow")
+        return "wallRow"
+
+    def sync(self):
+        check(lib().nz_sync(self._h), "sync")
+
+    # ---- read-back (solver.js:1012-1075)
+    @property
+    def state(self):
+        return {"A": "A", "B": "B"}
+
+    @property
+    def stepInfoTex(self):
+        return "step"
+
+    def _readRect(self, tex, x, y, w, h):
+        out = np.zeros(w * h * 4, np.float32)
+        check(lib().nz_read_rect(self._h, self.FIELDS[tex], x, y, w, h, _f(out)), "_readRect")
+        return out
+
+    def readStepInfo(self):
+        return self._readRect("step", 0, 0, 1, 1)
+
+    def readColumn(self, i):
+        return {"A": self._readRect("A", i, 0, 1, self.Nr), "B": self._readRect("B", i, 0, 1, self.Nr)}
+
+    def readCell(self, i, j):
+        return {"A": self._readRect("A", i, j, 1, 1), "B": self._readRect("B", i, j, 1, 1)}
+
+    def readAll(self):
+        return {"A": self._readRect("A", 0, 0, self.Nx, self.Nr), "B": self._readRect("B", 0, 0, self.Nx, self.Nr)}
+
+    def requestAsync(self, reqs, tag):
+        """reqs: [{name, tex, x, y, w, h}]; False while a read is in flight (solver.js:1028-1048)."""
+        if self._async is not None:
+            return False
+        arr = (abi.NzReadReq * len(reqs))()
+        for k, r in enumerate(reqs):
+            arr[k] = abi.NzReadReq(self.FIELDS[r["tex"]], r["x"], r["y"], r["w"], r["h"])
+        rc = lib().nz_read_async_begin(self._h, len(reqs), arr, int(tag))
+        if rc < 0:
+            check(rc, "requestAsync")
+        if rc == 0:
+            return False
+        self._async = {"reqs": reqs, "tag": tag}
+        return True
+
+    def pollAsync(self):
+        a = self._async
+        if a is None:
+            return None
+        out = np.zeros(_req_floats(a["reqs"]), np.float32)
+        tag = C.c_int32()
+        rc = lib().nz_read_async_poll(self._h, _f(out), C.byref(tag))
+        if rc < 0:
+            check(rc, "pollAsync")
+        if rc == 0:
+            return None
+        self._async = None
+        data, off = {}, 0
+        for r in a["reqs"]:
+            n = r["w"] * r["h"] * 4
+            data[r["name"]] = out[off:off + n]
+            off += n
+        return {"tag": a["tag"], "data": data, "probe": a.get("probe")}
+
+    def cancelAsync(self):
+        lib().nz_read_async_cancel(self._h)
+        self._async = None
+
+    # ---- build-defined extras
+    def setProbes(self, iExit, jExit, iThroat, iChamber, dx):
+        pr = abi.NzProbes(iExit, jExit, iThroat, iChamber, dx)
+        check(lib().nz_set_probes(self._h, C.byref(pr)), "setProbes")
+
+    def integrals(self, P):
+        out = abi.NzIntegrals()
+        check(lib().nz_integrals_read(self._h, C.byref(P), C.byref(out)), "integrals")
+        return {n: getattr(out, n) for n, _ in abi.NzIntegrals._fields_}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().nz_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def load_setup(solver, setup):
+    """The restart() sequence of app.js:229-252 for a SimSetup."""
+    solver.resize(setup.Nx, setup.Nr)
+    solver.setGeometry({"mask": setup.mask, "geoA": setup.geoA, "geoG": setup.geoG, "geoM": setup.geoM,
+                        "jWall": setup.jWall})
+    if setup.injector is not None:
+        solver.spray.setInjector({"tab": setup.injector})
+    A, B = setup.initial_state()
+    solver.setState(A, B)
+    solver.setProbes(setup.iExit, setup.jExit, setup.iThroat, setup.iChamber, setup.dx)
+    return A, B

@@ -1,0 +1,236 @@
+"""HIP solver against the CPU oracle (a pass-by-pass restatement of the reference's
+GLSL), called through the C ABI on the same seeded inputs.
+
+Tolerances (written here, per field):
+- FP32 GPU vs FP32 oracle after a short horizon: relative L2 of rho, u, v, p <= 1e-4
+  (the bar of the headline specification); step info to 1e-5.
+- FP64 GPU vs FP64 oracle: relative L2 <= 1e-9 (same arithmetic up to operation
+  order), which pins the kernel logic independently of FP32 rounding.
+The error of every FP32 result against the FP64 run is printed.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from rnozzle import configs
+from rnozzle._lib import lib
+from rnozzle.sim_setup import SimSetup
+from rnozzle.solver import GPUSolver, load_setup
+
+pytestmark = pytest.mark.gpu
+
+import oracle as orc  # noqa: E402
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module", autouse=True)
+def need_gpu():
+    if not lib().nz_device_available():
+        pytest.fail("no gfx950 device: the HIP path must run on an MI355X")
+
+
+def prims(A, B):
+    A = A.reshape(-1, 4).astype(np.float64)
+    B = B.reshape(-1, 4).astype(np.float64)
+    rho = np.maximum(A[:, 0], 1e-7)
+    u, v = A[:, 1] / rho, A[:, 2] / rho
+    G = np.maximum(B[:, 2], 0.5)
+    p = np.maximum((A[:, 3] - 0.5 * rho * (u * u + v * v)) / G, 1e-7)
+    return {"rho": rho, "u": u, "v": v, "p": p}
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def run_pair(setup, steps, prec=32, parcels=None, chunks=1):
+    """Advance the GPU solver and the oracle from the same inputs."""
+    say(f"[parity] Nx={setup.Nx} Nr={setup.Nr} steps={steps} prec={prec} parcels={parcels}")
+    g = GPUSolver(precision=prec)
+    if parcels:
+        g.resize(setup.Nx, setup.Nr)
+        g.spray.setCount(parcels)
+    A0, B0 = load_setup(g, setup) if not parcels else (None, None)
+    if parcels:
+        g.setGeometry({"mask": setup.mask, "geoA": setup.geoA, "geoG": setup.geoG, "geoM": setup.geoM, "jWall": setup.jWall})
+        g.spray.setInjector({"tab": setup.injector})
+        A0, B0 = setup.initial_state()
+        g.setState(A0, B0)
+    o = orc.Oracle(setup.Nx, setup.Nr, prec)
+    o.set_geometry(setup.mask, setup.geoA, setup.geoG, setup.geoM, setup.jWall)
+    if parcels:
+        o.set_parcels(parcels)
+    o.set_injector(setup.injector)
+    o.set_state(A0, B0)
+    for _ in range(chunks):
+        g.step(steps // chunks, setup.P)
+        o.step(steps // chunks, setup.P)
+    g.sync()
+    return g, o
+
+
+def compare_state(g, o, tol, label):
+    G = g.readAll()
+    Ao, Bo = o.state()
+    pg, po = prims(G["A"], G["B"]), prims(Ao, Bo)
+    errs = {k: rel_l2(pg[k], po[k]) for k in pg}
+    say(f"[parity] {label}: rel-L2 " + " ".join(f"{k}={v:.2e}" for k, v in errs.items()))
+    assert np.isfinite(G["A"]).all() and np.isfinite(G["B"]).all()
+    for k, v in errs.items():
+        assert v <= tol, f"{label}: {k} rel-L2 {v:.3e} > {tol}"
+    si_g, si_o = g.readStepInfo(), o.step_info()
+    np.testing.assert_allclose(si_g, si_o, rtol=max(tol, 1e-5))
+    return errs
+
+
+@pytest.fixture(scope="module")
+def cold():
+    return SimSetup(configs.cold_cone())
+
+
+def test_cold_cone_fp32(cold):
+    g, o = run_pair(cold, 200)
+    compare_state(g, o, 1e-4, "cold cone FP32 200 steps")
+    assert g.stepCount == 200
+
+
+def test_cold_cone_fp64_logic(cold):
+    g, o = run_pair(cold, 100, prec=64)
+    compare_state(g, o, 1e-9, "cold cone FP64 100 steps")
+
+
+def test_cold_cone_fp32_vs_fp64(cold):
+    """Error of the FP32 GPU result against the FP64 GPU run of the same kernels."""
+    g32, _ = run_pair(cold, 200)
+    g64 = GPUSolver(precision=64)
+    load_setup(g64, cold)
+    g64.step(200, cold.P)
+    a, b = g32.readAll(), g64.readAll()
+    p32, p64 = prims(a["A"], a["B"]), prims(b["A"], b["B"])
+    errs = {k: rel_l2(p32[k], p64[k]) for k in p32}
+    say("[parity] cold cone FP32 vs FP64 (200 steps): " + " ".join(f"{k}={v:.2e}" for k, v in errs.items()))
+    assert max(errs.values()) < 1e-3
+
+
+def test_inviscid_demo():
+    s = SimSetup(configs.apply(configs.cold_cone(), viscous=0))
+    assert not s.P.visc and not s.P.chem
+    g, o = run_pair(s, 150)
+    compare_state(g, o, 1e-4, "inviscid cold cone")
+
+
+def test_hot_bell_full_physics():
+    s = SimSetup(configs.hot_bell(Nx=192))
+    assert s.P.visc and s.P.chem
+    g, o = run_pair(s, 120, chunks=2)
+    compare_state(g, o, 1e-4, "hot bell 192 full physics")
+
+
+def test_hot_bell_fp64_logic():
+    s = SimSetup(configs.hot_bell(Nx=192))
+    g, o = run_pair(s, 80, prec=64)
+    compare_state(g, o, 1e-9, "hot bell 192 FP64")
+
+
+def test_injector_spray():
+    s = SimSetup(configs.kerolox_injector(Nx=160))
+    assert s.P.inj and s.injector is not None
+    g, o = run_pair(s, 60, parcels=4096)
+    compare_state(g, o, 1e-4, "kerolox injector 160 with spray")
+    PA, PB = g.spray.parcels()
+    oA, oB = o.parcels(4096)
+    alive = PB.reshape(-1, 4)[:, 1] > 0
+    assert alive.sum() > 0
+    assert np.array_equal(alive, oB.reshape(-1, 4)[:, 1] > 0)
+    np.testing.assert_allclose(PA.reshape(-1, 4)[alive], oA.reshape(-1, 4)[alive], rtol=1e-3, atol=1e-3)
+    st = g.spray.stats()
+    assert st["parcels"] == int(alive.sum())
+
+
+def test_injector_fp64_logic():
+    s = SimSetup(configs.kerolox_injector(Nx=160))
+    g, o = run_pair(s, 40, prec=64, parcels=4096)
+    compare_state(g, o, 1e-7, "kerolox injector FP64")
+
+
+def test_rescale_ambient(cold):
+    g, o = run_pair(cold, 60)
+    P2 = cold.compute_params(cold.cfg.Pc, cold.info.atmP)
+    g.rescaleAmbient(0.8, 0.95, 0.1, P2)
+    o.rescale(0.8, 0.95, 0.1, P2)
+    g.step(20, P2)
+    o.step(20, P2)
+    compare_state(g, o, 1e-4, "after rescaleAmbient")
+
+
+def test_readbacks_and_async(cold):
+    g, o = run_pair(cold, 50)
+    all_ = g.readAll()
+    A = all_["A"].reshape(cold.Nr, cold.Nx, 4)
+    col = g.readColumn(cold.iExit)
+    np.testing.assert_array_equal(col["A"].reshape(cold.Nr, 4), A[:, cold.iExit])
+    cell = g.readCell(cold.iChamber, 0)
+    np.testing.assert_array_equal(cell["A"], A[0, cold.iChamber])
+    # wall row (FS_WALLROW) and T2 (FS_VCELL) against the oracle
+    g.drawWallRow()
+    wr = g._readRect("wallRow", 0, 0, cold.Nx, 1)
+    np.testing.assert_allclose(wr, o.wall_row(), rtol=1e-4, atol=1e-5)
+    t2 = g._readRect("T2", 0, 0, cold.Nx, cold.Nr)
+    ot2 = o.t2()
+    assert rel_l2(t2.astype(np.float64), ot2.astype(np.float64)) < 1e-4
+    reqs = [{"name": "step", "tex": "step", "x": 0, "y": 0, "w": 1, "h": 1},
+            {"name": "colA", "tex": "A", "x": cold.iExit, "y": 0, "w": 1, "h": cold.Nr},
+            {"name": "wall", "tex": "wallRow", "x": 0, "y": 0, "w": cold.Nx, "h": 1}]
+    assert g.requestAsync(reqs, 7)
+    assert not g.requestAsync(reqs, 8)
+    res = None
+    for _ in range(100000):
+        res = g.pollAsync()
+        if res:
+            break
+    assert res and res["tag"] == 7
+    np.testing.assert_array_equal(res["data"]["colA"], col["A"])
+    np.testing.assert_array_equal(res["data"]["step"], g.readStepInfo())
+
+
+def test_integrals_match_host_sums(cold):
+    g, _ = run_pair(cold, 300)
+    P = cold.P
+    I = g.integrals(P)
+    col = g.readColumn(cold.iExit)
+    A, B = col["A"].reshape(-1, 4).astype(np.float64), col["B"].reshape(-1, 4).astype(np.float64)
+    dx = cold.dx
+    F = md = 0.0
+    Ax = cold.Ax.reshape(cold.Nr + 1, cold.Nx + 1)
+    for j in range(cold.jExit):
+        rho = max(A[j, 0], 1e-9)
+        u, v = A[j, 1] / rho, A[j, 2] / rho
+        G = B[j, 2]
+        p = max((A[j, 3] - 0.5 * rho * (u * u + v * v)) / G, 1e-9)
+        dA = 2 * np.pi * abs(Ax[j, cold.iExit + 1]) * dx * dx
+        F += (rho * u * u + p - P.amb[3]) * dA
+        md += B[j, 0] * u * dA
+    assert I["F"] == pytest.approx(F * 101325.0, rel=1e-9, abs=1e-9)
+    assert I["mdot"] == pytest.approx(md * 1.225 * np.sqrt(101325 / 1.225), rel=1e-6, abs=1e-12)
+    assert I["mass"] > 0 and I["residual"] >= 0
+
+
+def test_deterministic_rerun(cold):
+    g1, _ = run_pair(cold, 100)
+    g2, _ = run_pair(cold, 100)
+    a, b = g1.readAll(), g2.readAll()
+    assert np.array_equal(a["A"], b["A"]) and np.array_equal(a["B"], b["B"])
+
+
+def test_errors_are_raised():
+    g = GPUSolver()
+    with pytest.raises(Exception):
+        g.step(1, SimSetup(configs.cold_cone(Nx=64)).P)   # no resize / geometry
+    g.resize(64, 24)
+    with pytest.raises(Exception):
+        g._readRect("A", 60, 0, 10, 1)                   # outside the texture
