@@ -124,6 +124,9 @@ int nz_draw_wall_row(nz_solver* s);
 int nz_read_rect(nz_solver* s, int field, int x, int y, int w, int h, float* out);
 /* GPUSolver.readStepInfo() (solver.js:1072): out[4] */
 int nz_read_step_info(nz_solver* s, float* out);
+/* the whole state at the solver's precision, as doubles (no counterpart in the
+ * reference; the FP64 error reference reads through it): A, B are Nx*Nr*4 */
+int nz_read_state_f64(nz_solver* s, double* A, double* B);
 /* number of completed steps since setState (GPUSolver.stepCount) */
 int nz_step_count(const nz_solver* s, int64_t* out);
 /* block until all enqueued work is finished */

@@ -32,26 +32,29 @@ __device__ __forceinline__ int find_inst(const DevInst* __restrict__ I, int n, i
   return lo;
 }
 
-// FS_STEPINFO (solver.js:404-409): dt from the wave-speed max of the current state
-__device__ __forceinline__ void step_info(const Consts& c, const InstState& s, float* cur) {
-  const float w = fmaxf(c.wFloor, __uint_as_float(s.wsnap));
-  float dt = fminf(c.cfl / w, c.dtMax);
-  if (s.last[0] > 0.f) dt = fminf(dt, s.last[0] * 1.2f);
+// FS_STEPINFO (solver.js:404-409): dt from the wave-speed max of the current state,
+// in the arithmetic type of the state
+template <typename R>
+__device__ __forceinline__ void step_info(const Consts& c, const InstState& s, R* cur) {
+  const R w = fmax(R(c.wFloor), R(__longlong_as_double((long long)s.wsnap)));
+  R dt = fmin(R(c.cfl) / w, R(c.dtMax));
+  const R lastDt = R(s.last[0]), lastT = R(s.last[2]);
+  if (lastDt > R(0)) dt = fmin(dt, lastDt * R(1.2f));
   cur[0] = dt;
-  cur[1] = s.last[2];
-  cur[2] = s.last[2] + dt;
+  cur[1] = lastT;
+  cur[2] = lastT + dt;
   cur[3] = w;
 }
 
 template <int NT>
-__device__ __forceinline__ float block_max(float v, float* red) {
-  for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+__device__ __forceinline__ double block_max(double v, double* red) {
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   if (lane == 0) red[wv] = v;
   __syncthreads();
-  float m = 0.f;
+  double m = 0.0;
   if (threadIdx.x == 0) {
-    for (int k = 0; k < NT / 64; ++k) m = fmaxf(m, red[k]);
+    for (int k = 0; k < NT / 64; ++k) m = fmax(m, red[k]);
   }
   return m;  // valid on thread 0
 }
@@ -165,7 +168,7 @@ struct StageSmem {
       R Vr[7][TY + 1][TX];
     } vis;
   } u;
-  float red[16];
+  double red[16];
   double redd[16];
   int last;
 };
@@ -191,17 +194,17 @@ __global__ void __launch_bounds__(TX* TY) k_stage(const DevInst* __restrict__ in
   const R* __restrict__ Uin = STAGE == 1 ? U0 : U1;
 
   // time step of this step: FS_STEPINFO at the start of the step
-  float cur[4];
+  R cur[4];
   if (STAGE == 1) {
-    step_info(c, *st, cur);
+    step_info<R>(c, *st, cur);
     if (lt_ == 0 && tid == 0 && active) { st->cur[0] = cur[0]; st->cur[1] = cur[1]; st->cur[2] = cur[2]; st->cur[3] = cur[3]; }
   } else {
-    cur[0] = st->cur[0]; cur[1] = st->cur[1]; cur[2] = st->cur[2]; cur[3] = st->cur[3];
+    cur[0] = R(st->cur[0]); cur[1] = R(st->cur[1]); cur[2] = R(st->cur[2]); cur[3] = R(st->cur[3]);
   }
-  const R dt = R(cur[0]);
-  const R tsel = R(STAGE == 1 ? cur[1] : cur[2]);
+  const R dt = cur[0];
+  const R tsel = STAGE == 1 ? cur[1] : cur[2];
 
-  float wloc = 0.f;
+  double wloc = 0.0;
   double rloc = 0.0;
   if (active) {
     // ---- 1. primitive tile with ghost halo (FS_PRIM)
@@ -358,7 +361,7 @@ __global__ void __launch_bounds__(TX* TY) k_stage(const DevInst* __restrict__ in
             ub = R(0);
             if (mi[m] == 0) {
               const R u0 = Uin[7 * Ns + (long long)mj[m] * Nx];
-              const bool on = STAGE == 2 && R(cur[2]) > R(c.uBar[2]);
+              const bool on = STAGE == 2 && cur[2] > R(c.uBar[2]);
               ub = on ? fmin(u0 + fmin(dt * R(c.uBar[0]), R(1)) * (vx - u0), R(c.uBar[1])) : u0;
             }
           }
@@ -569,7 +572,7 @@ __global__ void __launch_bounds__(TX* TY) k_stage(const DevInst* __restrict__ in
       }
       if (STAGE == 2) {
         const Prim<R> qn = to_prim<R>(A[0], A[1], A[2], A[3], B[0], B[1], B[2], ubarM[0]);
-        wloc = (float)(fabs(qn.u) + fabs(qn.v) + R(2) * sound_speed(qn));
+        wloc = (double)(fabs(qn.u) + fabs(qn.v) + R(2) * sound_speed(qn));
         // slaves are fluid cells holding the same state
         const double dr = (double)(A[0] - rho_old) / fmax((double)dt, 1e-30);
         rloc = dr * dr;
@@ -578,10 +581,10 @@ __global__ void __launch_bounds__(TX* TY) k_stage(const DevInst* __restrict__ in
   }
 
   if (STAGE == 2) {
-    const float wb = block_max<NT>(wloc, sm.red);
+    const double wb = block_max<NT>(wloc, sm.red);
     const double rb = block_sum<NT>(rloc, sm.redd);
     if (tid == 0 && active) {
-      atomicMax(&st->wacc, __float_as_uint(wb));
+      atomicMax(&st->wacc, (unsigned long long)__double_as_longlong(wb));
       unsafeAtomicAdd(&st->resAcc, rb);
     }
     // last block to finish closes the step for every instance in the launch
@@ -596,7 +599,7 @@ __global__ void __launch_bounds__(TX* TY) k_stage(const DevInst* __restrict__ in
         if (s->flags & 1u) continue;
         s->last[0] = s->cur[0]; s->last[1] = s->cur[1]; s->last[2] = s->cur[2]; s->last[3] = s->cur[3];
         s->wsnap = __hip_atomic_load(&s->wacc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        s->wacc = 0u;
+        s->wacc = 0ull;
         s->steps += 1;
         const double ra = __hip_atomic_load(&s->resAcc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         s->resLast = sqrt(ra / fmax(s->nGroups, 1.0)) * (double)J.c.resScale;
@@ -631,115 +634,114 @@ __global__ void __launch_bounds__(256) k_spray(const DevInst* __restrict__ insts
   const long long Ns = I.Ns;
   const R* __restrict__ U1 = static_cast<const R*>(I.U1);
   float* __restrict__ pc = I.parc;
-  const float dt = st->cur[0];
+  const R dt = R(st->cur[0]);
   const uint32_t head = st->head, seed = st->seed;
-  const float ax_ = pc[idx], ay = pc[NP + idx], az = pc[2 * NP + idx], aw = pc[3 * NP + idx];
-  const float bx = pc[4 * NP + idx], by = pc[5 * NP + idx], bz = pc[6 * NP + idx], bw = pc[7 * NP + idx];
+  const R ax_ = pc[idx], ay = pc[NP + idx], az = pc[2 * NP + idx], aw = pc[3 * NP + idx];
+  const R bx = pc[4 * NP + idx], by = pc[5 * NP + idx], bz = pc[6 * NP + idx], bw = pc[7 * NP + idx];
   int rel = idx - (int)head;
   if (rel < 0) rel += NP;
-  float o[8] = {ax_, ay, az, aw, bx, by, bz, bw};
-  float dmo = 0.f, dpx = 0.f, dpy = 0.f;
+  R o[8] = {ax_, ay, az, aw, bx, by, bz, bw};
+  R dmo = 0, dpx = 0, dpy = 0;
   const int perStep = 4;
   if (rel < perStep) {
     // new parcel; the liquid left in the replaced one goes to the gas
     const uint32_t id = (uint32_t)idx ^ (seed * 2654435761u);
-    const int tt = (int)(rnd(id, 1u) * 255.999f);
+    const int tt = (int)(R(rnd(id, 1u)) * R(255.999f));
     const float* e1 = I.tab + tt * 4;
     const float* e2 = I.tab + (256 + tt) * 4;
     const float* e3 = I.tab + (512 + tt) * 4;
-    const float ang = e1[2] + (rnd(id, 2u) - 0.5f) * 2.0f * e2[2];
-    const float r = fmaxf(e1[1] + (rnd(id, 3u) - 0.5f) * e2[3], 0.05f);
-    const float d = e2[1] * expf((rnd(id, 4u) - 0.5f) * 1.4f);
-    o[0] = e1[0]; o[1] = r; o[2] = e1[3] * cosf(ang); o[3] = e1[3] * sinf(ang);
-    o[4] = d; o[5] = c.sprayMhat * dt / (float)perStep; o[6] = e2[0]; o[7] = e3[0];
-    if (by > 0.f) { dmo = by; dpx = by * az; dpy = by * aw; }
-  } else if (by > 0.f) {
-    const int gi = min(max((int)floorf(ax_), 0), Nx - 1), gj = min(max((int)floorf(ay), 0), Nr - 1);
+    const R ang = R(e1[2]) + (R(rnd(id, 2u)) - R(0.5)) * R(2) * R(e2[2]);
+    const R r = fmax(R(e1[1]) + (R(rnd(id, 3u)) - R(0.5)) * R(e2[3]), R(0.05f));
+    const R d = R(e2[1]) * exp((R(rnd(id, 4u)) - R(0.5)) * R(1.4f));
+    o[0] = e1[0]; o[1] = r; o[2] = R(e1[3]) * cos(ang); o[3] = R(e1[3]) * sin(ang);
+    o[4] = d; o[5] = R(c.sprayMhat) * dt / R(perStep); o[6] = e2[0]; o[7] = e3[0];
+    if (by > R(0)) { dmo = by; dpx = by * az; dpy = by * aw; }
+  } else if (by > R(0)) {
+    const int gi = min(max((int)floor(ax_), 0), Nx - 1), gj = min(max((int)floor(ay), 0), Nr - 1);
     const long long kg = (long long)gj * Nx + gi;
     if (I.code[kg] & C_SOLID) {
       // born inside solid hardware: walk out into the flow
-      o[0] = ax_ + 1.0f; o[1] = fmaxf(ay - 0.5f, 0.05f); o[2] = fmaxf(az, 0.f); o[3] = aw;
+      o[0] = ax_ + R(1); o[1] = fmax(ay - R(0.5), R(0.05f)); o[2] = fmax(az, R(0)); o[3] = aw;
     } else {
-      const float* pr = bz < 0.5f ? c.fuelVec : c.oxVec;
-      float vx = az, vy = aw;
-      const float m = by, d = bx, dts = dt * c.sprayTScale;
-      float dn = d, mn = m;
+      const float* pr = bz < R(0.5) ? c.fuelVec : c.oxVec;
+      const R pr0 = pr[0], pr1 = pr[1], pr2 = pr[2], pr3 = pr[3];
+      R vx = az, vy = aw;
+      const R m = by, d = bx, dts = dt * R(c.sprayTScale);
+      R dn = d, mn = m;
       const Prim<R> w = load_prim<R>(U1, Ns, kg);
       {
-        const float wr = (float)w.r, wu = (float)w.u, wv = (float)w.v, wp = (float)w.p;
-        const float rm = (1.0f - (float)w.Z) * c.gas[0] + (float)w.F * c.Rs[0] + (float)w.O * c.Rs[1] +
-                         fmaxf((float)w.Z - (float)w.F - (float)w.O, 0.f) * c.Rs[2];
-        const float T = wp / (wr * rm) * c.gas[2];
-        const float rhoG = wr * 1.225f;
-        const float rvx = wu - vx, rvy = wv - vy;
-        const float Urel = sqrtf(rvx * rvx + rvy * rvy) * 287.6f + 1e-3f;
-        const float mu = 1.458e-6f * T * sqrtf(T) / (T + 110.4f);
-        const float Re = rhoG * Urel * d / mu;
-        const float invTau = 18.0f * mu * (1.0f + 0.15f * powf(fminf(Re, 1000.0f), 0.687f)) / (pr[0] * d * d);
-        const float ex = expf(-dts * invTau);
-        vx = wu + (vx - wu) * ex;
-        vy = wv + (vy - wv) * ex;
-        const float cp = rm * 287.05f * ((float)w.G + 1.0f);
-        const float Bn = fmaxf(cp * (T - pr[1]), 0.f) / pr[2];
-        const float K = 8.0f * (mu / 0.7f) / pr[0] * logf(1.0f + Bn) * (1.0f + 0.27f * sqrtf(Re));
-        const float d2 = d * d - K * dts;
-        dn = d2 > 1e-12f ? sqrtf(d2) : 0.f;
-        const float f = dn / d;
-        mn = dn > 0.f ? m * f * f * f : 0.f;
-        const float we = rhoG * Urel * Urel * dn / pr[3];
-        if (we > 12.0f && dn > 0.f) {
-          const float dst = 12.0f * pr[3] / (rhoG * Urel * Urel);
-          const float tb = 3.0f * dn / Urel * sqrtf(pr[0] / rhoG);
-          dn = mixR(dn, fmaxf(dst, 2e-6f), 1.0f - expf(-dts / tb));
+        const R rm = (R(1) - w.Z) * R(c.gas[0]) + w.F * R(c.Rs[0]) + w.O * R(c.Rs[1]) + fmax(w.Z - w.F - w.O, R(0)) * R(c.Rs[2]);
+        const R T = w.p / (w.r * rm) * R(c.gas[2]);
+        const R rhoG = w.r * R(1.225f);
+        const R rvx = w.u - vx, rvy = w.v - vy;
+        const R Urel = sqrt(rvx * rvx + rvy * rvy) * R(287.6f) + R(1e-3f);
+        const R mu = R(1.458e-6f) * T * sqrt(T) / (T + R(110.4f));
+        const R Re = rhoG * Urel * d / mu;
+        const R invTau = R(18) * mu * (R(1) + R(0.15f) * pow(fmin(Re, R(1000)), R(0.687f))) / (pr0 * d * d);
+        const R ex = exp(-dts * invTau);
+        vx = w.u + (vx - w.u) * ex;
+        vy = w.v + (vy - w.v) * ex;
+        const R cp = rm * R(287.05f) * (w.G + R(1));
+        const R Bn = fmax(cp * (T - pr1), R(0)) / pr2;
+        const R K = R(8) * (mu / R(0.7f)) / pr0 * log(R(1) + Bn) * (R(1) + R(0.27f) * sqrt(Re));
+        const R d2 = d * d - K * dts;
+        dn = d2 > R(1e-12f) ? sqrt(d2) : R(0);
+        const R f = dn / d;
+        mn = dn > R(0) ? m * f * f * f : R(0);
+        const R we = rhoG * Urel * Urel * dn / pr3;
+        if (we > R(12) && dn > R(0)) {
+          const R dst = R(12) * pr3 / (rhoG * Urel * Urel);
+          const R tb = R(3) * dn / Urel * sqrt(pr0 / rhoG);
+          dn = mixR(dn, fmax(dst, R(2e-6f)), R(1) - exp(-dts / tb));
         }
       }
-      float xi = bw;
-      if (xi > 0.f && ax_ >= xi) {
+      R xi = bw;
+      if (xi > R(0) && ax_ >= xi) {
         const uint32_t id = (uint32_t)idx ^ (seed * 2654435761u);
-        const float sp = sqrtf(vx * vx + vy * vy), phi = (rnd(id, 5u) - 0.5f) * 2.0f * c.fan;
-        vx = sp * cosf(phi);
-        vy = sp * sinf(phi);
-        dn *= 0.25f;
-        xi = 0.f;
+        const R sp = sqrt(vx * vx + vy * vy), phi = (R(rnd(id, 5u)) - R(0.5)) * R(2) * R(c.fan);
+        vx = sp * cos(phi);
+        vy = sp * sin(phi);
+        dn *= R(0.25);
+        xi = R(0);
       }
-      float x = ax_ + vx * dt, r = ay + vy * dt;
-      if (r < 0.f) { r = -r; vy = -vy; }
-      if (x < 0.f) { x = -x; vx = fabsf(vx) * 0.3f; }
-      auto wallAt = [&](float px, float pr_) {
-        const int qi = (int)floorf(px), qj = (int)floorf(pr_);
+      R x = ax_ + vx * dt, r = ay + vy * dt;
+      if (r < R(0)) { r = -r; vy = -vy; }
+      if (x < R(0)) { x = -x; vx = fabs(vx) * R(0.3f); }
+      auto wallAt = [&](R px, R pr_) {
+        const int qi = (int)floor(px), qj = (int)floor(pr_);
         if (qi < 0 || qj < 0 || qi >= Nx || qj >= Nr) return false;
         return (I.code[(long long)qj * Nx + qi] & C_SOLID) != 0;
       };
       if (wallAt(x, r)) {
         const bool hx = wallAt(x, ay), hr = wallAt(ax_, r);
-        if (hx) { vx *= -0.3f; x = ax_; }
-        if (hr) { vy *= -0.3f; r = ay; }
-        if (!hx && !hr) { x = ax_; r = ay; vx *= -0.3f; vy *= -0.3f; }
+        if (hx) { vx *= R(-0.3f); x = ax_; }
+        if (hr) { vy *= R(-0.3f); r = ay; }
+        if (!hx && !hr) { x = ax_; r = ay; vx *= R(-0.3f); vy *= R(-0.3f); }
       }
-      const float dm = m - mn;
+      const R dm = m - mn;
       dmo = dm;
       dpx = mn * (az - vx) + dm * az;
       dpy = mn * (aw - vy) + dm * aw;
-      if (x >= (float)Nx || r >= (float)Nr || mn <= 0.f) {
-        o[0] = -1.f; o[1] = -1.f; o[2] = 0.f; o[3] = 0.f; o[4] = 0.f; o[5] = 0.f; o[6] = 0.f; o[7] = 0.f;
+      if (x >= R(Nx) || r >= R(Nr) || mn <= R(0)) {
+        o[0] = R(-1); o[1] = R(-1); o[2] = 0; o[3] = 0; o[4] = 0; o[5] = 0; o[6] = 0; o[7] = 0;
       } else {
         o[0] = x; o[1] = r; o[2] = vx; o[3] = vy; o[4] = dn; o[5] = mn; o[6] = bz; o[7] = xi;
       }
     }
   }
-  for (int q = 0; q < 8; ++q) pc[q * NP + idx] = o[q];
+  for (int q = 0; q < 8; ++q) pc[q * NP + idx] = (float)o[q];
   // deposit into the host of the cell the parcel started in
-  if (!(ax_ < 0.f) && !(dmo == 0.f && dpx == 0.f && dpy == 0.f)) {
-    int gi = min(max((int)floorf(ax_), 0), Nx - 1), gj = min(max((int)floorf(ay), 0), Nr - 1);
+  if (!(ax_ < R(0)) && !(dmo == R(0) && dpx == R(0) && dpy == R(0))) {
+    int gi = min(max((int)floor(ax_), 0), Nx - 1), gj = min(max((int)floor(ay), 0), Nr - 1);
     const uint32_t cg = I.code[(long long)gj * Nx + gi];
     gi += code_hdx(cg);
     gj += code_hdy(cg);
     const long long h = (long long)gj * Nx + gi;
     float* __restrict__ src = I.src;
-    unsafeAtomicAdd(&src[h], dmo * (1.0f - bz));
-    unsafeAtomicAdd(&src[Ns + h], dmo * bz);
-    unsafeAtomicAdd(&src[2 * Ns + h], dpx);
-    unsafeAtomicAdd(&src[3 * Ns + h], dpy);
+    unsafeAtomicAdd(&src[h], (float)(dmo * (R(1) - bz)));
+    unsafeAtomicAdd(&src[Ns + h], (float)(dmo * bz));
+    unsafeAtomicAdd(&src[2 * Ns + h], (float)dpx);
+    unsafeAtomicAdd(&src[3 * Ns + h], (float)dpy);
   }
 }
 
@@ -747,7 +749,7 @@ __global__ void __launch_bounds__(256) k_spray(const DevInst* __restrict__ insts
 // wave-speed max of the current state + step-info refresh (after setState / rescale)
 template <typename R>
 __global__ void __launch_bounds__(256) k_cfl_init(const DevInst* __restrict__ insts, int ninst, unsigned* __restrict__ done) {
-  __shared__ float red[16];
+  __shared__ double red[16];
   __shared__ int last;
   const int tid = threadIdx.x;
   // grid-stride over the cells of every instance: instance by block ranges
@@ -757,14 +759,14 @@ __global__ void __launch_bounds__(256) k_cfl_init(const DevInst* __restrict__ in
   const DevInst& I = insts[ii];
   const long long N = (long long)I.Nx * I.Nr;
   const R* U = static_cast<const R*>(I.U0);
-  float w = 0.f;
+  double w = 0.0;
   for (long long k = (long long)(blockIdx.x - I.cblk0) * 256 + tid; k < N; k += (long long)I.ncblk * 256) {
     if (I.code[k] & C_SOLID) continue;
     const Prim<R> q = load_prim<R>(U, I.Ns, k);
-    w = fmaxf(w, (float)(fabs(q.u) + fabs(q.v) + R(2) * sound_speed(q)));
+    w = fmax(w, (double)(fabs(q.u) + fabs(q.v) + R(2) * sound_speed(q)));
   }
-  const float wb = block_max<256>(w, red);
-  if (tid == 0) atomicMax(&I.st->wacc, __float_as_uint(wb));
+  const double wb = block_max<256>(w, red);
+  if (tid == 0) atomicMax(&I.st->wacc, (unsigned long long)__double_as_longlong(wb));
   __threadfence();
   if (tid == 0) last = (atomicAdd(done, 1u) == gridDim.x - 1) ? 1 : 0;
   __syncthreads();
@@ -773,7 +775,7 @@ __global__ void __launch_bounds__(256) k_cfl_init(const DevInst* __restrict__ in
     for (int n = tid; n < ninst; n += 256) {
       InstState* s = insts[n].st;
       s->wsnap = __hip_atomic_load(&s->wacc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      s->wacc = 0u;
+      s->wacc = 0ull;
     }
     if (tid == 0) *done = 0u;
   }
@@ -832,6 +834,17 @@ __global__ void k_pack(const R* __restrict__ U, long long Ns, int Nx, int p0, in
   const int i = x + k % w, j = y + k / w;
   const long long c = (long long)j * Nx + i;
   for (int q = 0; q < 4; ++q) out[(long long)k * 4 + q] = (float)U[(p0 + q) * Ns + c];
+}
+// the last step info as four floats (readStepInfo)
+__global__ void k_step_info(const InstState* __restrict__ st, float* __restrict__ out) {
+  if (threadIdx.x < 4) out[threadIdx.x] = (float)st->last[threadIdx.x];
+}
+// interleave the full state in double (FP64 error analysis read-back)
+template <typename R>
+__global__ void k_pack_f64(const R* __restrict__ U, long long Ns, long long N, double* __restrict__ A, double* __restrict__ B) {
+  const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= N) return;
+  for (int q = 0; q < 4; ++q) { A[k * 4 + q] = (double)U[q * Ns + k]; B[k * 4 + q] = (double)U[(4 + q) * Ns + k]; }
 }
 // de-interleave RGBA A/B into the state planes (setState upload)
 template <typename R>
@@ -985,6 +998,16 @@ hipError_t launch_unpack(bool fp64, void* U, long long Ns, long long N, const fl
   const dim3 g((unsigned)((N + 255) / 256)), b(256);
   if (fp64) hipLaunchKernelGGL((k_unpack<double>), g, b, 0, s, static_cast<double*>(U), Ns, N, A, B);
   else hipLaunchKernelGGL((k_unpack<float>), g, b, 0, s, static_cast<float*>(U), Ns, N, A, B);
+  return hipGetLastError();
+}
+hipError_t launch_step_info(const InstState* st, float* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_step_info, dim3(1), dim3(64), 0, s, st, out);
+  return hipGetLastError();
+}
+hipError_t launch_pack_f64(bool fp64, const void* U, long long Ns, long long N, double* A, double* B, hipStream_t s) {
+  const dim3 g((unsigned)((N + 255) / 256)), b(256);
+  if (fp64) hipLaunchKernelGGL((k_pack_f64<double>), g, b, 0, s, static_cast<const double*>(U), Ns, N, A, B);
+  else hipLaunchKernelGGL((k_pack_f64<float>), g, b, 0, s, static_cast<const float*>(U), Ns, N, A, B);
   return hipGetLastError();
 }
 hipError_t launch_spray_reset(float* parc, int NP, float* src, long long Ns, hipStream_t s) {

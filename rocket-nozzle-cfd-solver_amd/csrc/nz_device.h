@@ -52,10 +52,10 @@ struct Consts {
 
 // per-instance mutable scalars kept on the device between launches
 struct InstState {
-  float cur[4];    // step info of the step in progress (dt, t_old, t_new, w)
-  float last[4];   // step info of the last completed step (readStepInfo)
-  uint32_t wacc;   // max wave speed being accumulated (float bits, >= 0)
-  uint32_t wsnap;  // max wave speed of the current state
+  double cur[4];   // step info of the step in progress (dt, t_old, t_new, w), in the state's precision
+  double last[4];  // step info of the last completed step (readStepInfo)
+  unsigned long long wacc;   // max wave speed being accumulated (double bits, >= 0)
+  unsigned long long wsnap;  // max wave speed of the current state
   uint32_t head;   // spray ring head
   uint32_t seed;   // spray seed
   long long steps; // completed steps since setState

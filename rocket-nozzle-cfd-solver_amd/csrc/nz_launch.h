@@ -20,6 +20,8 @@ hipError_t launch_wall_row(bool fp64, const DevInst& I, const float* jw, float* 
 hipError_t launch_t2(bool fp64, const DevInst& I, float* out, hipStream_t s);
 hipError_t launch_pack(bool fp64, const void* U, long long Ns, int Nx, int p0, int x, int y, int w, int h, float* out, hipStream_t s);
 hipError_t launch_unpack(bool fp64, void* U, long long Ns, long long N, const float* A, const float* B, hipStream_t s);
+hipError_t launch_step_info(const InstState* st, float* out, hipStream_t s);
+hipError_t launch_pack_f64(bool fp64, const void* U, long long Ns, long long N, double* A, double* B, hipStream_t s);
 hipError_t launch_spray_reset(float* parc, int NP, float* src, long long Ns, hipStream_t s);
 hipError_t launch_integrals(bool fp64, const DevInst& I, int iExit, int jExit, int iThroat, int iChamber, double* out, hipStream_t s);
 

@@ -497,8 +497,7 @@ static long long enqueue_rect(nz_solver* s, int field, int x, int y, int w, int 
     if (nzk::launch_pack(fp64, s->U0, s->Ns, Nx, field == NZ_FIELD_A ? 0 : 4, x, y, w, h, dst, s->stream) != hipSuccess)
       return nz_fail(-10, "read: pack failed");
   } else if (field == NZ_FIELD_STEP) {
-    if (hipMemcpyAsync(dst, &s->st->last[0], 16, hipMemcpyDeviceToDevice, s->stream) != hipSuccess)
-      return nz_fail(-10, "read: step info copy failed");
+    if (nzk::launch_step_info(s->st, dst, s->stream) != hipSuccess) return nz_fail(-10, "read: step info failed");
   } else if (field == NZ_FIELD_WALLROW) {
     for (int r = 0; r < h; ++r)
       if (hipMemcpyAsync(dst + (size_t)r * w * 4, s->wallrow + (size_t)x * 4, (size_t)w * 16, hipMemcpyDeviceToDevice, s->stream) != hipSuccess)
@@ -529,6 +528,19 @@ int nz_read_rect(nz_solver* s, int field, int x, int y, int w, int h, float* out
 }
 
 int nz_read_step_info(nz_solver* s, float* out) { return nz_read_rect(s, NZ_FIELD_STEP, 0, 0, 1, 1, out); }
+
+int nz_read_state_f64(nz_solver* s, double* A, double* B) {
+  if (int rc = check_solver(s, "nz_read_state_f64")) return rc;
+  if (!A || !B) return nz_fail(-1, "nz_read_state_f64: null output");
+  const long long N = (long long)s->Nx * s->Nr;
+  if (int rc = ensure_staging(s, (size_t)N * 16)) return rc;
+  double* d = reinterpret_cast<double*>(s->dstage);
+  HIPCHK(nzk::launch_pack_f64(s->prec == 64, s->U0, s->Ns, N, d, d + N * 4, s->stream), "pack f64");
+  HIPCHK(hipMemcpyAsync(A, d, (size_t)N * 32, hipMemcpyDeviceToHost, s->stream), "copy A");
+  HIPCHK(hipMemcpyAsync(B, d + N * 4, (size_t)N * 32, hipMemcpyDeviceToHost, s->stream), "copy B");
+  HIPCHK(hipStreamSynchronize(s->stream), "nz_read_state_f64");
+  return 0;
+}
 
 int nz_step_count(const nz_solver* s, int64_t* out) {
   if (!s || !out) return nz_fail(-1, "nz_step_count: null argument");

@@ -124,6 +124,7 @@ SIGNATURES = {
     "nz_draw_wall_row": (C.c_int, [vp]),
     "nz_read_rect": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, fp]),
     "nz_read_step_info": (C.c_int, [vp, fp]),
+    "nz_read_state_f64": (C.c_int, [vp, dp, dp]),
     "nz_step_count": (C.c_int, [vp, P(C.c_int64)]),
     "nz_sync": (C.c_int, [vp]),
     "nz_read_async_begin": (C.c_int, [vp, C.c_int, P(NzReadReq), C.c_int32]),

@@ -159,6 +159,14 @@ class GPUSolver:
     def readAll(self):
         return {"A": self._readRect("A", 0, 0, self.Nx, self.Nr), "B": self._readRect("B", 0, 0, self.Nx, self.Nr)}
 
+    def readAllF64(self):
+        """The state at the solver's own precision (FP64 runs), RGBA-interleaved doubles."""
+        n = self.Nx * self.Nr * 4
+        A, B = np.zeros(n), np.zeros(n)
+        dp = C.POINTER(C.c_double)
+        check(lib().nz_read_state_f64(self._h, A.ctypes.data_as(dp), B.ctypes.data_as(dp)), "readAllF64")
+        return {"A": A, "B": B}
+
     def requestAsync(self, reqs, tag):
         """reqs: [{name, tex, x, y, w, h}]; False while a read is in flight (solver.js:1028-1048)."""
         if self._async is not None:
