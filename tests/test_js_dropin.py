@@ -1,0 +1,75 @@
+"""The JavaScript drop-in (js/gpu_solver.js over the Node-API addon) runs the reference's
+app.js call sequence and reproduces the Python/C-ABI path bit for bit."""
+import json
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+from rnozzle import configs
+from rnozzle.sim_setup import SimSetup
+from rnozzle.solver import GPUSolver, load_setup
+
+PKG = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "rocket-nozzle-cfd-solver_amd")
+ADDON = os.path.join(PKG, "build_out", "nozzle_napi.node")
+NODE = shutil.which("node")
+COLD = {"Nx": 256, "aspect": 0.375, "prop": "coldgas", "Tc": 300, "gamma": 1.4, "Mw": 28, "qAb": 0, "sAir": 1, "Dt": 0.02,
+        "De": 0.04, "type": "cone", "coneAngle": 15, "convAngle": 30, "CR": 6.25, "chamberLen": 1.4, "Pc": 10e5,
+        "domainL": 0.3, "cfl": 0.5, "ambP": 1e5, "ambT": 300, "turbulence": 0}
+
+needs_node = pytest.mark.skipif(NODE is None or not os.path.exists(ADDON), reason="node or the addon is not available")
+
+
+def node(script, *args, timeout=120):
+    r = subprocess.run([NODE, os.path.join(PKG, "js", script), *args], capture_output=True, text=True, timeout=timeout)
+    assert r.returncode == 0, r.stderr[-3000:]
+    return r.stdout
+
+
+@needs_node
+def test_addon_setup_matches_python():
+    out = node_setup(COLD)
+    s = SimSetup(configs.cold_cone())
+    assert (out["Nx"], out["Nr"]) == (s.Nx, s.Nr)
+    assert out["dx"] == s.dx
+    h = float(np.sum(np.fmod(s.geoA.astype(np.float64) * np.arange(s.geoA.size), 97)))
+    assert out["geoHash"] == pytest.approx(h, rel=1e-12)
+
+
+def node_setup(cfg):
+    code = ("const m=require(process.argv[1]);const s=m.buildSimSetup(JSON.parse(process.argv[2]));"
+            "let h=0;for(let k=0;k<s.geo.geoA.length;k++)h+=(s.geo.geoA[k]*k)%97;"
+            "console.log(JSON.stringify({Nx:s.Nx,Nr:s.Nr,dx:s.dx,geoHash:h}));")
+    r = subprocess.run([NODE, "-e", code, os.path.join(PKG, "js", "gpu_solver.js"), json.dumps(cfg)],
+                       capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stderr
+    return json.loads(r.stdout)
+
+
+@needs_node
+def test_constructor_throws_like_reference_without_device():
+    code = ("const m=require(process.argv[1]);if(m.deviceAvailable()){console.log('device');process.exit(0)}"
+            "try{new m.GPUSolver(null);console.log('no-throw')}catch(e){console.log('threw:'+(e instanceof Error))}")
+    r = subprocess.run([NODE, "-e", code, os.path.join(PKG, "js", "gpu_solver.js")], capture_output=True, text=True, timeout=60)
+    assert r.stdout.strip() in ("device", "threw:true")
+
+
+@needs_node
+@pytest.mark.gpu
+def test_dropin_sequence_matches_c_abi_path():
+    out = json.loads(node("dropin_check.js", json.dumps(COLD), "6", "10"))
+    assert out["finite"] and out["stepCount"] == 160      # 60 frame steps + 100 speed-loop steps
+    assert out["requested"] >= 1 and out["polled"] >= 1
+    assert out["thrustDevice"] == pytest.approx(out["thrustJS"], rel=1e-6, abs=1e-9)
+    # the same sequence through the Python host gives the identical state
+    s = SimSetup(configs.cold_cone())
+    g = GPUSolver()
+    load_setup(g, s)
+    g.step(60, s.P)
+    g.step(100, s.P)
+    a = g.readAll()
+    assert float(np.sum(a["A"].astype(np.float64))) == pytest.approx(out["sumA"], rel=1e-12)
+    assert list(g.readStepInfo()) == pytest.approx(out["stepInfo"], rel=1e-7)
+    print("drop-in ms/step (self-test loop):", out["msPerStep"])
