@@ -73,6 +73,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--workload", default="hot_bell")
     ap.add_argument("--nx", type=int, default=0, help="override the grid width (other sizes are parity cases)")
+    ap.add_argument("--set", action="append", default=[], help="config override key=value (e.g. wallModel=stair)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     args = ap.parse_args()
@@ -88,6 +89,16 @@ def main():
 
     make = getattr(configs, args.workload)
     cfg = make(Nx=args.nx) if args.nx else make()
+    over = {}
+    for kv in args.set:
+        k, v = kv.split("=", 1)
+        try:
+            v = float(v) if "." in v or "e" in v.lower() else int(v)
+        except ValueError:
+            pass
+        over[k] = v
+    if over:
+        cfg = configs.apply(cfg, over)
     s = SimSetup(cfg)
     g = GPUSolver()
     load_setup(g, s)
@@ -118,7 +129,8 @@ def main():
         "metric": "cell-updates/s", "value": value, "unit": "cell-updates/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "fp32", "data": "synthetic (setup from the workload's geometry)",
-        "config": {"workload": args.workload, "grid": f"{s.Nx}x{s.Nr}", "cells": cells, "physics": bool(s.P.visc),
+        "config": {"workload": args.workload, "overrides": over, "grid": f"{s.Nx}x{s.Nr}", "cells": cells,
+                   "merged_cells": int(s.info.nMerged), "physics": bool(s.P.visc),
                    "afterburning": bool(s.P.chem), "spray": spray, "cfl": round(float(s.P.cfl), 3)},
         "roofline": roof,
         "kernel_ms": {"stage1": ms[0], "spray": ms[1], "stage2": ms[2], "events_span": ms[3]},

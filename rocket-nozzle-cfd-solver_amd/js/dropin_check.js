@@ -45,6 +45,12 @@ for (let f = 0; f < frames; f++) {
 }
 while (solver._async) { const r = solver.pollAsync(); if (r) { polled++; lastAsync = r; } }
 
+// the self-test speed loop: 10 x step(10) closed by a synchronous step-info read
+const t0 = process.hrtime.bigint();
+for (let k = 0; k < 10; k++) solver.step(10, P);
+solver.readStepInfo();
+const ms = Number(process.hrtime.bigint() - t0) / 1e6 / 100;
+
 // readback(): synchronous step info, exit column and chamber cell
 const si = solver.readStepInfo();
 const col = solver.readColumn(setup.iExit);
@@ -62,12 +68,6 @@ for (let j = 0; j < setup.jExit; j++) {
   F += (rho * u * u + p - P.amb[3]) * dA;
 }
 const integ = solver.integrals(P);
-
-// the self-test speed loop: 10 x step(10) closed by a synchronous step-info read
-const t0 = process.hrtime.bigint();
-for (let k = 0; k < 10; k++) solver.step(10, P);
-solver.readStepInfo();
-const ms = Number(process.hrtime.bigint() - t0) / 1e6 / 100;
 
 const out = {
   Nx: setup.Nx, Nr: setup.Nr, stepCount: solver.stepCount, stepInfo: Array.from(si),

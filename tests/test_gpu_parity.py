@@ -122,7 +122,7 @@ def test_cold_cone_fp32_vs_fp64(cold):
 
 
 def test_inviscid_demo():
-    s = SimSetup(configs.apply(configs.cold_cone(), viscous=0))
+    s = SimSetup(configs.apply(configs.cold_cone(), {"viscous": 0}))
     assert not s.P.visc and not s.P.chem
     g, o = run_pair(s, 150)
     compare_state(g, o, 1e-4, "inviscid cold cone")
