@@ -72,83 +72,88 @@ __device__ __forceinline__ double block_sum(double v, double* red) {
 }
 
 // ------------------------------------------------------------------------------
-// global-memory evaluation of one cell's own net inviscid flux (r-weighted face
-// areas), used for merged-group members that lie outside the LDS tile
-template <typename R>
-__device__ void own_flux_global(const DevInst& I, const R* U, R t, int ci, int cj, R* L, R* LY, R& LO) {
-  const int Nx = I.Nx;
-  const Consts& c = I.c;
-  // faces: west (ci, cj), east (ci+1, cj) in x; south (ci, cj), north (ci, cj+1) in r
-  for (int f = 0; f < 4; ++f) {
-    const bool dirR = f >= 2;
-    const int fi = ci + (f == 1 ? 1 : 0), fj = cj + (f == 3 ? 1 : 0);
-    const float area = dirR ? I.ar[(long long)fj * Nx + fi] : I.ax[(long long)fj * (Nx + 1) + fi];
-    if (area == 0.f || (dirR && fj == 0)) continue;
-    const int dx = dirR ? 0 : 1, dy = dirR ? 1 : 0, tx = dirR ? 1 : 0, ty = dirR ? 0 : 1;
-    Prim<R> c0 = prim_at<R>(I, U, fi - 2 * dx, fj - 2 * dy, t);
-    Prim<R> c1 = prim_at<R>(I, U, fi - dx, fj - dy, t);
-    Prim<R> c2 = prim_at<R>(I, U, fi, fj, t);
-    Prim<R> c3 = prim_at<R>(I, U, fi + dx, fj + dy, t);
-    const bool s1 = c1.r < R(0), s2 = c2.r < R(0);
-    if (s1 && s2) continue;
-    R hllw = R(0);
-    if (c.hllBlend > 0.f && !(s1 || s2)) {
-      const Prim<R> a1 = prim_at<R>(I, U, fi - dx + tx, fj - dy + ty, t), b1 = prim_at<R>(I, U, fi - dx - tx, fj - dy - ty, t);
-      const Prim<R> a2 = prim_at<R>(I, U, fi + tx, fj + ty, t), b2 = prim_at<R>(I, U, fi - tx, fj - ty, t);
-      hllw = hll_weight<R>(c1.p, c2.p, a1.r < R(0) ? c1.p : a1.p, b1.r < R(0) ? c1.p : b1.p,
-                           a2.r < R(0) ? c2.p : a2.p, b2.r < R(0) ? c2.p : b2.p, R(c.hllBlend));
-    }
-    if (dirR) {
-      R s;
-      s = c0.u; c0.u = c0.v; c0.v = s;
-      s = c1.u; c1.u = c1.v; c1.v = s;
-      s = c2.u; c2.u = c2.v; c2.v = s;
-      s = c3.u; c3.u = c3.v; c3.v = s;
-    }
-    R F[4], FY[4], FO;
-    face_flux<R>(c0, c1, c2, c3, hllw, c.order2 && area > 0.f, F, FY, FO);
-    if (dirR) { const R s = F[1]; F[1] = F[2]; F[2] = s; }
-    const R a = R(fabsf(area));
-    const R sg = (f == 1 || f == 3) ? R(-1) : R(1);  // east/north leave, west/south enter
-    for (int q = 0; q < 4; ++q) { L[q] += sg * a * F[q]; LY[q] += sg * a * FY[q]; }
-    LO += sg * a * FO;
+// One face's inviscid flux times |area| (FS_FLUX_X/R for the face, as memberFlux
+// weights it), with primitives from `getP(i, j)`: the LDS tile, or the tile with a
+// global fall-back for faces of merged cells that lie outside it.
+template <typename R, class GetP>
+__device__ __forceinline__ void face_flux_w(const DevInst& I, bool dirR, int fi, int fj, GetP getP, R* F, R* FY, R& FO) {
+  for (int q = 0; q < 4; ++q) { F[q] = R(0); FY[q] = R(0); }
+  FO = R(0);
+  const int Nx = I.Nx, Nr = I.Nr;
+  const bool inDom = dirR ? (fi >= 0 && fi < Nx && fj >= 0 && fj <= Nr) : (fi >= 0 && fi <= Nx && fj >= 0 && fj < Nr);
+  if (!inDom) return;
+  const float area = dirR ? I.ar[(long long)fj * Nx + fi] : I.ax[(long long)fj * (Nx + 1) + fi];
+  if (area == 0.f || (dirR && fj == 0)) return;
+  const int dx = dirR ? 0 : 1, dy = dirR ? 1 : 0, tx = dy, ty = dx;
+  Prim<R> c0 = getP(fi - 2 * dx, fj - 2 * dy), c1 = getP(fi - dx, fj - dy);
+  Prim<R> c2 = getP(fi, fj), c3 = getP(fi + dx, fj + dy);
+  const bool s1 = c1.r < R(0), s2 = c2.r < R(0);
+  if (s1 && s2) return;
+  R hllw = R(0);
+  if (I.c.hllBlend > 0.f && !(s1 || s2)) {
+    const Prim<R> a1 = getP(fi - dx + tx, fj - dy + ty), b1 = getP(fi - dx - tx, fj - dy - ty);
+    const Prim<R> a2 = getP(fi + tx, fj + ty), b2 = getP(fi - tx, fj - ty);
+    hllw = hll_weight<R>(c1.p, c2.p, a1.r < R(0) ? c1.p : a1.p, b1.r < R(0) ? c1.p : b1.p,
+                         a2.r < R(0) ? c2.p : a2.p, b2.r < R(0) ? c2.p : b2.p, R(I.c.hllBlend));
   }
+  if (dirR) {
+    R t;
+    t = c0.u; c0.u = c0.v; c0.v = t;
+    t = c1.u; c1.u = c1.v; c1.v = t;
+    t = c2.u; c2.u = c2.v; c2.v = t;
+    t = c3.u; c3.u = c3.v; c3.v = t;
+  }
+  face_flux<R>(c0, c1, c2, c3, hllw, I.c.order2 && area > 0.f, F, FY, FO);
+  if (dirR) { const R t = F[1]; F[1] = F[2]; F[2] = t; }
+  const R a = R(fabsf(area));
+  for (int q = 0; q < 4; ++q) { F[q] *= a; FY[q] *= a; }
+  FO *= a;
+}
+
+// One face's viscous flux times |area| (FS_VFLUX_X/R); getT(i, j, T1, T2) gives the
+// per-cell gradients / viscosities of an interior fluid cell.
+template <typename R, class GetP, class GetT>
+__device__ __forceinline__ void vface_w(const DevInst& I, bool dirR, int fi, int fj, GetP getP, GetT getT, R* F, R* FY) {
+  for (int q = 0; q < 4; ++q) F[q] = R(0);
+  for (int q = 0; q < 3; ++q) FY[q] = R(0);
+  const int Nx = I.Nx, Nr = I.Nr;
+  const bool on = dirR ? (fi >= 0 && fi < Nx && fj > 0 && fj < Nr) : (fi > 0 && fi < Nx && fj >= 0 && fj < Nr);
+  if (!on) return;
+  const float area = dirR ? I.ar[(long long)fj * Nx + fi] : I.ax[(long long)fj * (Nx + 1) + fi];
+  if (area == 0.f) return;
+  const int k1i = dirR ? fi : fi - 1, k1j = dirR ? fj - 1 : fj;
+  const Prim<R> w1 = getP(k1i, k1j), w2 = getP(fi, fj);
+  if (w1.r < R(0) || w2.r < R(0)) return;
+  R t1[4], m1[4], t2[4], m2[4];
+  getT(k1i, k1j, t1, m1);
+  getT(fi, fj, t2, m2);
+  visc_face<R>(dirR, fj, w1, w2, t1, t2, m1, m2, F, FY);
+  const R a = R(fabsf(area));
+  for (int q = 0; q < 4; ++q) F[q] *= a;
+  for (int q = 0; q < 3; ++q) FY[q] *= a;
+}
+
+// T1/T2 of an interior cell from primitives given by getP (zero for walls)
+template <typename R, class GetP>
+__device__ __forceinline__ void vcell_at(const DevInst& I, GetP getP, R dt, int i, int j, R* T1, R* T2) {
+  for (int q = 0; q < 4; ++q) { T1[q] = R(0); T2[q] = R(0); }
+  const Prim<R> w0 = getP(i, j);
+  if (w0.r < R(0)) return;
+  vcell<R>(w0, getP(i + 1, j), getP(i - 1, j), getP(i, j + 1), getP(i, j - 1), i, j, R(I.lt[(long long)j * I.Nx + i]), dt,
+           I.c, T1, T2);
 }
 
 // T1/T2 of an interior cell from global memory
 template <typename R>
 __device__ void vcell_global(const DevInst& I, const R* U, R t, R dt, int i, int j, R* T1, R* T2) {
-  const Prim<R> w0 = prim_at<R>(I, U, i, j, t);
-  for (int q = 0; q < 4; ++q) { T1[q] = R(0); T2[q] = R(0); }
-  if (w0.r < R(0)) return;
-  const Prim<R> wE = prim_at<R>(I, U, i + 1, j, t), wW = prim_at<R>(I, U, i - 1, j, t);
-  const Prim<R> wN = prim_at<R>(I, U, i, j + 1, t), wS = prim_at<R>(I, U, i, j - 1, t);
-  vcell<R>(w0, wE, wW, wN, wS, i, j, R(I.lt[(long long)j * I.Nx + i]), dt, I.c, T1, T2);
+  auto g = [&](int a, int b) { return prim_at<R>(I, U, a, b, t); };
+  vcell_at<R>(I, g, dt, i, j, T1, T2);
 }
 
-// a cell's own net viscous flux from global memory
-template <typename R>
-__device__ void own_vflux_global(const DevInst& I, const R* U, R t, R dt, int ci, int cj, R* L, R* LY) {
-  const int Nx = I.Nx, Nr = I.Nr;
-  for (int f = 0; f < 4; ++f) {
-    const bool dirR = f >= 2;
-    const int fi = ci + (f == 1 ? 1 : 0), fj = cj + (f == 3 ? 1 : 0);
-    const float area = dirR ? I.ar[(long long)fj * Nx + fi] : I.ax[(long long)fj * (Nx + 1) + fi];
-    if (area == 0.f) continue;
-    if (dirR ? (fj == 0 || fj >= Nr) : (fi == 0 || fi >= Nx)) continue;
-    const int k1i = dirR ? fi : fi - 1, k1j = dirR ? fj - 1 : fj;
-    const Prim<R> w1 = prim_at<R>(I, U, k1i, k1j, t), w2 = prim_at<R>(I, U, fi, fj, t);
-    if (w1.r < R(0) || w2.r < R(0)) continue;
-    R t1[4], m1[4], t2[4], m2[4], F[4], FY[3];
-    vcell_global<R>(I, U, t, dt, k1i, k1j, t1, m1);
-    vcell_global<R>(I, U, t, dt, fi, fj, t2, m2);
-    visc_face<R>(dirR, fj, w1, w2, t1, t2, m1, m2, F, FY);
-    const R a = R(fabsf(area));
-    const R sg = (f == 1 || f == 3) ? R(-1) : R(1);
-    for (int q = 0; q < 4; ++q) L[q] += sg * a * F[q];
-    for (int q = 0; q < 3; ++q) LY[q] += sg * a * FY[q];
-  }
-}
+// extra face of a merged cell outside the tile: (face x | dir << 31, face y)
+__device__ __forceinline__ bool ex_dir(uint2 e) { return (e.x >> 31) != 0; }
+__device__ __forceinline__ int ex_i(uint2 e) { return (int)(e.x & 0x7fffffffu); }
+__device__ __forceinline__ int ex_j(uint2 e) { return (int)e.y; }
 
 // ------------------------------------------------------------------------------
 template <typename R, int TX, int TY>
@@ -168,6 +173,11 @@ struct StageSmem {
       R Vr[7][TY + 1][TX];
     } vis;
   } u;
+  static constexpr int MAXE = 64;                 // extra faces of merged cells outside the tile
+  uint2 ex[MAXE];
+  R Fe[9][MAXE];
+  R Ve[7][MAXE];
+  int nEx;
   double red[16];
   double redd[16];
   int last;
@@ -207,71 +217,82 @@ __global__ void __launch_bounds__(TX* TY) k_stage(const DevInst* __restrict__ in
   double wloc = 0.0;
   double rloc = 0.0;
   if (active) {
-    // ---- 1. primitive tile with ghost halo (FS_PRIM)
+    // ---- 1. primitive tile with ghost halo (FS_PRIM); this tile's extra faces
     for (int idx = tid; idx < S::PW * S::PH; idx += NT) {
       const int li = idx % S::PW, lj = idx / S::PW;
       const Prim<R> q = prim_at<R>(I, Uin, i0 - 2 + li, j0 - 2 + lj, tsel);
       sm.P[0][lj][li] = q.r; sm.P[1][lj][li] = q.u; sm.P[2][lj][li] = q.v; sm.P[3][lj][li] = q.p;
       sm.P[4][lj][li] = q.Z; sm.P[5][lj][li] = q.F; sm.P[6][lj][li] = q.G; sm.P[7][lj][li] = q.O;
     }
+    if (tid == 0) {
+      const int e0 = I.exStart[lt_], e1 = I.exStart[lt_ + 1];
+      sm.nEx = min(e1 - e0, S::MAXE);
+    }
     __syncthreads();
+    const int nEx = sm.nEx;
+    for (int e = tid; e < nEx; e += NT) sm.ex[e] = I.exFaces[I.exStart[lt_] + e];
     auto P_at = [&](int li, int lj) {
       Prim<R> q;
       q.r = sm.P[0][lj][li]; q.u = sm.P[1][lj][li]; q.v = sm.P[2][lj][li]; q.p = sm.P[3][lj][li];
       q.Z = sm.P[4][lj][li]; q.F = sm.P[5][lj][li]; q.G = sm.P[6][lj][li]; q.O = sm.P[7][lj][li];
       return q;
     };
+    // primitives by global cell index: the LDS tile, which always covers in-tile faces
+    auto tileP = [&](int gi, int gj) { return P_at(gi - (i0 - 2), gj - (j0 - 2)); };
+    // ... and with a global fall-back, for faces of merged cells outside the tile
+    auto anyP = [&](int gi, int gj) {
+      const int li = gi - (i0 - 2), lj = gj - (j0 - 2);
+      if (li >= 0 && li < S::PW && lj >= 0 && lj < S::PH) return P_at(li, lj);
+      return prim_at<R>(I, Uin, gi, gj, tsel);
+    };
+    __syncthreads();
 
-    // ---- 2. each face flux once (FS_FLUX_X / FS_FLUX_R), times |area|
-    for (int f = tid; f < S::NFX + S::NFR; f += NT) {
-      const bool dirR = f >= S::NFX;
-      const int g = dirR ? f - S::NFX : f;
-      const int fx = dirR ? g % TX : g % (TX + 1);
-      const int fy = dirR ? g / TX : g / (TX + 1);
-      const int gi = i0 + fx, gj = j0 + fy;
-      R F[4] = {0, 0, 0, 0}, FY[4] = {0, 0, 0, 0}, FO = 0;
-      float area = 0.f;
-      const bool inDom = dirR ? (gi < Nx && gj <= Nr) : (gi <= Nx && gj < Nr);
-      if (inDom) area = dirR ? I.ar[(long long)gj * Nx + gi] : I.ax[(long long)gj * (Nx + 1) + gi];
-      if (area != 0.f && !(dirR && gj == 0)) {
-        const int dx = dirR ? 0 : 1, dy = dirR ? 1 : 0;
-        const int li = fx + 2, lj = fy + 2;
-        Prim<R> c0 = P_at(li - 2 * dx, lj - 2 * dy), c1 = P_at(li - dx, lj - dy);
-        Prim<R> c2 = P_at(li, lj), c3 = P_at(li + dx, lj + dy);
-        const bool s1 = c1.r < R(0), s2 = c2.r < R(0);
-        if (!(s1 && s2)) {
-          R hllw = R(0);
-          if (c.hllBlend > 0.f && !(s1 || s2)) {
-            const int tx = dy, ty = dx;
-            const R pa1 = sm.P[0][lj - dy + ty][li - dx + tx] < R(0) ? c1.p : sm.P[3][lj - dy + ty][li - dx + tx];
-            const R pb1 = sm.P[0][lj - dy - ty][li - dx - tx] < R(0) ? c1.p : sm.P[3][lj - dy - ty][li - dx - tx];
-            const R pa2 = sm.P[0][lj + ty][li + tx] < R(0) ? c2.p : sm.P[3][lj + ty][li + tx];
-            const R pb2 = sm.P[0][lj - ty][li - tx] < R(0) ? c2.p : sm.P[3][lj - ty][li - tx];
-            hllw = hll_weight<R>(c1.p, c2.p, pa1, pb1, pa2, pb2, R(c.hllBlend));
-          }
-          if (dirR) {
-            R s;
-            s = c0.u; c0.u = c0.v; c0.v = s;
-            s = c1.u; c1.u = c1.v; c1.v = s;
-            s = c2.u; c2.u = c2.v; c2.v = s;
-            s = c3.u; c3.u = c3.v; c3.v = s;
-          }
-          face_flux<R>(c0, c1, c2, c3, hllw, c.order2 && area > 0.f, F, FY, FO);
-          if (dirR) { const R s = F[1]; F[1] = F[2]; F[2] = s; }
-          const R a = R(fabsf(area));
-          for (int q = 0; q < 4; ++q) { F[q] *= a; FY[q] *= a; }
-          FO *= a;
+    // ---- 2. each face flux once (FS_FLUX_X / FS_FLUX_R), times |area|; extra faces last
+    for (int f = tid; f < S::NFX + S::NFR + nEx; f += NT) {
+      R F[4], FY[4], FO;
+      if (f < S::NFX + S::NFR) {
+        const bool dirR = f >= S::NFX;
+        const int g = dirR ? f - S::NFX : f;
+        const int fx = dirR ? g % TX : g % (TX + 1);
+        const int fy = dirR ? g / TX : g / (TX + 1);
+        face_flux_w<R>(I, dirR, i0 + fx, j0 + fy, tileP, F, FY, FO);
+        if (dirR) {
+          for (int q = 0; q < 4; ++q) { sm.u.inv.Fr[q][fy][fx] = F[q]; sm.u.inv.Fr[4 + q][fy][fx] = FY[q]; }
+          sm.u.inv.Fr[8][fy][fx] = FO;
+        } else {
+          for (int q = 0; q < 4; ++q) { sm.u.inv.Fx[q][fy][fx] = F[q]; sm.u.inv.Fx[4 + q][fy][fx] = FY[q]; }
+          sm.u.inv.Fx[8][fy][fx] = FO;
         }
-      }
-      if (dirR) {
-        for (int q = 0; q < 4; ++q) { sm.u.inv.Fr[q][fy][fx] = F[q]; sm.u.inv.Fr[4 + q][fy][fx] = FY[q]; }
-        sm.u.inv.Fr[8][fy][fx] = FO;
       } else {
-        for (int q = 0; q < 4; ++q) { sm.u.inv.Fx[q][fy][fx] = F[q]; sm.u.inv.Fx[4 + q][fy][fx] = FY[q]; }
-        sm.u.inv.Fx[8][fy][fx] = FO;
+        const int e = f - S::NFX - S::NFR;
+        const uint2 ef = sm.ex[e];
+        face_flux_w<R>(I, ex_dir(ef), ex_i(ef), ex_j(ef), anyP, F, FY, FO);
+        for (int q = 0; q < 4; ++q) { sm.Fe[q][e] = F[q]; sm.Fe[4 + q][e] = FY[q]; }
+        sm.Fe[8][e] = FO;
       }
     }
     __syncthreads();
+
+    // a face's weighted inviscid flux: in-tile array, extra-face array, or computed here
+    auto invFace = [&](bool dirR, int fi, int fj, R* F9) {
+      const int fx = fi - i0, fy = fj - j0;
+      if (!dirR && fx >= 0 && fx <= TX && fy >= 0 && fy < TY) {
+        for (int q = 0; q < 9; ++q) F9[q] = sm.u.inv.Fx[q][fy][fx];
+        return;
+      }
+      if (dirR && fx >= 0 && fx < TX && fy >= 0 && fy <= TY) {
+        for (int q = 0; q < 9; ++q) F9[q] = sm.u.inv.Fr[q][fy][fx];
+        return;
+      }
+      for (int e = 0; e < nEx; ++e) {
+        const uint2 ef = sm.ex[e];
+        if (ex_dir(ef) == dirR && ex_i(ef) == fi && ex_j(ef) == fj) {
+          for (int q = 0; q < 9; ++q) F9[q] = sm.Fe[q][e];
+          return;
+        }
+      }
+      face_flux_w<R>(I, dirR, fi, fj, anyP, F9, F9 + 4, F9[8]);
+    };
 
     // ---- 3. merged-group finite-volume update (FS_UPDATE)
     const int tx = tid % TX, ty = tid / TX;
@@ -302,16 +323,17 @@ __global__ void __launch_bounds__(TX* TY) k_stage(const DevInst* __restrict__ in
       }
       LO = -(sm.u.inv.Fx[8][ty][tx + 1] - sm.u.inv.Fx[8][ty][tx]) - (sm.u.inv.Fr[8][ty + 1][tx] - sm.u.inv.Fr[8][ty][tx]);
       for (int m = 1; m < nm; ++m) {
-        const int sx = mi[m] - i0, sy = mj[m] - j0;
-        if (sx >= 0 && sx < TX && sy >= 0 && sy < TY) {
-          for (int q = 0; q < 4; ++q) {
-            L[q] += -(sm.u.inv.Fx[q][sy][sx + 1] - sm.u.inv.Fx[q][sy][sx]) - (sm.u.inv.Fr[q][sy + 1][sx] - sm.u.inv.Fr[q][sy][sx]);
-            LY[q] += -(sm.u.inv.Fx[4 + q][sy][sx + 1] - sm.u.inv.Fx[4 + q][sy][sx]) - (sm.u.inv.Fr[4 + q][sy + 1][sx] - sm.u.inv.Fr[4 + q][sy][sx]);
-          }
-          LO += -(sm.u.inv.Fx[8][sy][sx + 1] - sm.u.inv.Fx[8][sy][sx]) - (sm.u.inv.Fr[8][sy + 1][sx] - sm.u.inv.Fr[8][sy][sx]);
-        } else {
-          own_flux_global<R>(I, Uin, tsel, mi[m], mj[m], L, LY, LO);
+        // the slave's own net flux: west/south faces enter, east/north faces leave
+        R Fw[9], Fe_[9], Fs[9], Fn[9];
+        invFace(false, mi[m], mj[m], Fw);
+        invFace(false, mi[m] + 1, mj[m], Fe_);
+        invFace(true, mi[m], mj[m], Fs);
+        invFace(true, mi[m], mj[m] + 1, Fn);
+        for (int q = 0; q < 4; ++q) {
+          L[q] += -(Fe_[q] - Fw[q]) - (Fn[q] - Fs[q]);
+          LY[q] += -(Fe_[4 + q] - Fw[4 + q]) - (Fn[4 + q] - Fs[4 + q]);
         }
+        LO += -(Fe_[8] - Fw[8]) - (Fn[8] - Fs[8]);
       }
       gg = I.gg[k];
       R a[4], b[4];
@@ -386,43 +408,64 @@ __global__ void __launch_bounds__(TX* TY) k_stage(const DevInst* __restrict__ in
         for (int q = 0; q < 4; ++q) { sm.u.vis.T[q][rj][ri] = T1[q]; sm.u.vis.T[4 + q][rj][ri] = T2[q]; }
       }
       __syncthreads();
+      // per-cell T1/T2: the LDS ring tile, else computed from primitives
+      auto anyT = [&](int gi, int gj, R* T1, R* T2) {
+        const int ri = gi - (i0 - 1), rj = gj - (j0 - 1);
+        if (ri >= 0 && ri < S::TW && rj >= 0 && rj < S::THh) {
+          for (int q = 0; q < 4; ++q) { T1[q] = sm.u.vis.T[q][rj][ri]; T2[q] = sm.u.vis.T[4 + q][rj][ri]; }
+          return;
+        }
+        vcell_at<R>(I, anyP, dt, gi, gj, T1, T2);
+      };
+      auto tileT = [&](int gi, int gj, R* T1, R* T2) {
+        const int ri = gi - (i0 - 1), rj = gj - (j0 - 1);
+        for (int q = 0; q < 4; ++q) { T1[q] = sm.u.vis.T[q][rj][ri]; T2[q] = sm.u.vis.T[4 + q][rj][ri]; }
+      };
       if (c.visc) {
-        for (int f = tid; f < S::NFX + S::NFR; f += NT) {
-          const bool dirR = f >= S::NFX;
-          const int g = dirR ? f - S::NFX : f;
-          const int fx = dirR ? g % TX : g % (TX + 1);
-          const int fy = dirR ? g / TX : g / (TX + 1);
-          const int gi = i0 + fx, gj = j0 + fy;
-          R F[4] = {0, 0, 0, 0}, FY[3] = {0, 0, 0};
-          bool on = dirR ? (gi < Nx && gj > 0 && gj < Nr) : (gi > 0 && gi < Nx && gj < Nr);
-          float area = 0.f;
-          if (on) area = dirR ? I.ar[(long long)gj * Nx + gi] : I.ax[(long long)gj * (Nx + 1) + gi];
-          if (on && area != 0.f) {
-            const int li = fx + 2, lj = fy + 2;
-            const int l1i = dirR ? li : li - 1, l1j = dirR ? lj - 1 : lj;
-            const Prim<R> w1 = P_at(l1i, l1j), w2 = P_at(li, lj);
-            if (!(w1.r < R(0) || w2.r < R(0))) {
-              R t1[4], m1[4], t2[4], m2[4];
-              for (int q = 0; q < 4; ++q) {
-                t1[q] = sm.u.vis.T[q][l1j - 1][l1i - 1]; m1[q] = sm.u.vis.T[4 + q][l1j - 1][l1i - 1];
-                t2[q] = sm.u.vis.T[q][lj - 1][li - 1]; m2[q] = sm.u.vis.T[4 + q][lj - 1][li - 1];
-              }
-              visc_face<R>(dirR, gj, w1, w2, t1, t2, m1, m2, F, FY);
-              const R a = R(fabsf(area));
-              for (int q = 0; q < 4; ++q) F[q] *= a;
-              for (int q = 0; q < 3; ++q) FY[q] *= a;
+        for (int f = tid; f < S::NFX + S::NFR + nEx; f += NT) {
+          R F[4], FY[3];
+          if (f < S::NFX + S::NFR) {
+            const bool dirR = f >= S::NFX;
+            const int g = dirR ? f - S::NFX : f;
+            const int fx = dirR ? g % TX : g % (TX + 1);
+            const int fy = dirR ? g / TX : g / (TX + 1);
+            vface_w<R>(I, dirR, i0 + fx, j0 + fy, tileP, tileT, F, FY);
+            if (dirR) {
+              for (int q = 0; q < 4; ++q) sm.u.vis.Vr[q][fy][fx] = F[q];
+              for (int q = 0; q < 3; ++q) sm.u.vis.Vr[4 + q][fy][fx] = FY[q];
+            } else {
+              for (int q = 0; q < 4; ++q) sm.u.vis.Vx[q][fy][fx] = F[q];
+              for (int q = 0; q < 3; ++q) sm.u.vis.Vx[4 + q][fy][fx] = FY[q];
             }
-          }
-          if (dirR) {
-            for (int q = 0; q < 4; ++q) sm.u.vis.Vr[q][fy][fx] = F[q];
-            for (int q = 0; q < 3; ++q) sm.u.vis.Vr[4 + q][fy][fx] = FY[q];
           } else {
-            for (int q = 0; q < 4; ++q) sm.u.vis.Vx[q][fy][fx] = F[q];
-            for (int q = 0; q < 3; ++q) sm.u.vis.Vx[4 + q][fy][fx] = FY[q];
+            const int e = f - S::NFX - S::NFR;
+            const uint2 ef = sm.ex[e];
+            vface_w<R>(I, ex_dir(ef), ex_i(ef), ex_j(ef), anyP, anyT, F, FY);
+            for (int q = 0; q < 4; ++q) sm.Ve[q][e] = F[q];
+            for (int q = 0; q < 3; ++q) sm.Ve[4 + q][e] = FY[q];
           }
         }
         __syncthreads();
       }
+      auto visFace = [&](bool dirR, int fi, int fj, R* F7) {
+        const int fx = fi - i0, fy = fj - j0;
+        if (!dirR && fx >= 0 && fx <= TX && fy >= 0 && fy < TY) {
+          for (int q = 0; q < 7; ++q) F7[q] = sm.u.vis.Vx[q][fy][fx];
+          return;
+        }
+        if (dirR && fx >= 0 && fx < TX && fy >= 0 && fy <= TY) {
+          for (int q = 0; q < 7; ++q) F7[q] = sm.u.vis.Vr[q][fy][fx];
+          return;
+        }
+        for (int e = 0; e < nEx; ++e) {
+          const uint2 ef = sm.ex[e];
+          if (ex_dir(ef) == dirR && ex_i(ef) == fi && ex_j(ef) == fj) {
+            for (int q = 0; q < 7; ++q) F7[q] = sm.Ve[q][e];
+            return;
+          }
+        }
+        vface_w<R>(I, dirR, fi, fj, anyP, anyT, F7, F7 + 4);
+      };
       if (host) {
         const R rho = qS.r, p = qS.p, G = qS.G, velx = qS.u, vely = qS.v;
         const R V = fmax(R(gg.x), R(1e-6));
@@ -432,15 +475,13 @@ __global__ void __launch_bounds__(TX* TY) k_stage(const DevInst* __restrict__ in
         for (int q = 0; q < 4; ++q) { T1h[q] = sm.u.vis.T[q][ty + 1][tx + 1]; T2h[q] = sm.u.vis.T[4 + q][ty + 1][tx + 1]; }
         if (c.visc) {
           for (int m = 0; m < nm; ++m) {
-            const int sx = mi[m] - i0, sy = mj[m] - j0;
-            if (sx >= 0 && sx < TX && sy >= 0 && sy < TY) {
-              for (int q = 0; q < 4; ++q)
-                L[q] += -(sm.u.vis.Vx[q][sy][sx + 1] - sm.u.vis.Vx[q][sy][sx]) - (sm.u.vis.Vr[q][sy + 1][sx] - sm.u.vis.Vr[q][sy][sx]);
-              for (int q = 0; q < 3; ++q)
-                LY[q] += -(sm.u.vis.Vx[4 + q][sy][sx + 1] - sm.u.vis.Vx[4 + q][sy][sx]) - (sm.u.vis.Vr[4 + q][sy + 1][sx] - sm.u.vis.Vr[4 + q][sy][sx]);
-            } else {
-              own_vflux_global<R>(I, U1, tsel, dt, mi[m], mj[m], L, LY);
-            }
+            R Fw[7], Fe_[7], Fs[7], Fn[7];
+            visFace(false, mi[m], mj[m], Fw);
+            visFace(false, mi[m] + 1, mj[m], Fe_);
+            visFace(true, mi[m], mj[m], Fs);
+            visFace(true, mi[m], mj[m] + 1, Fn);
+            for (int q = 0; q < 4; ++q) L[q] += -(Fe_[q] - Fw[q]) - (Fn[q] - Fs[q]);
+            for (int q = 0; q < 3; ++q) LY[q] += -(Fe_[4 + q] - Fw[4 + q]) - (Fn[4 + q] - Fs[4 + q]);
           }
           // axisymmetric hoop stress (solver.js:587-588)
           const R r = V / fmax(R(gg.y), R(1e-6)), vor = vely / r;
@@ -581,35 +622,43 @@ __global__ void __launch_bounds__(TX* TY) k_stage(const DevInst* __restrict__ in
   }
 
   if (STAGE == 2) {
+    // per-tile partials, reduced by k_finalize (no same-address atomics across workgroups)
     const double wb = block_max<NT>(wloc, sm.red);
     const double rb = block_sum<NT>(rloc, sm.redd);
-    if (tid == 0 && active) {
-      atomicMax(&st->wacc, (unsigned long long)__double_as_longlong(wb));
-      unsafeAtomicAdd(&st->resAcc, rb);
-    }
-    // last block to finish closes the step for every instance in the launch
-    __threadfence();
-    if (tid == 0) sm.last = (atomicAdd(done, 1u) == gridDim.x - 1) ? 1 : 0;
-    __syncthreads();
-    if (sm.last) {
-      __threadfence();
-      for (int n = tid; n < ninst; n += NT) {
-        const DevInst& J = insts[n];
-        InstState* s = J.st;
-        if (s->flags & 1u) continue;
-        s->last[0] = s->cur[0]; s->last[1] = s->cur[1]; s->last[2] = s->cur[2]; s->last[3] = s->cur[3];
-        s->wsnap = __hip_atomic_load(&s->wacc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        s->wacc = 0ull;
-        s->steps += 1;
-        const double ra = __hip_atomic_load(&s->resAcc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        s->resLast = sqrt(ra / fmax(s->nGroups, 1.0)) * (double)J.c.resScale;
-        s->resAcc = 0.0;
-        if (J.c.spray) {
-          s->head = (s->head + 4u) % (uint32_t)J.c.NP;
-          s->seed = s->seed + 1u;
-        }
-      }
-      if (tid == 0) *done = 0u;
+    if (tid == 0) I.part[lt_] = make_double2(active ? wb : 0.0, active ? rb : 0.0);
+  }
+  (void)done;
+}
+
+// Closes the step of every instance (one workgroup per instance): wave-speed max
+// and residual over its tiles, step info of the completed step (FS_STEPINFO's
+// input for the next one), spray ring head and seed (spray.js:295-296).
+__global__ void __launch_bounds__(256) k_finalize(const DevInst* __restrict__ insts, int ninst) {
+  __shared__ double red[16], redd[16];
+  const int n = blockIdx.x;
+  if (n >= ninst) return;
+  const DevInst& J = insts[n];
+  InstState* s = J.st;
+  if (s->flags & 1u) return;
+  const int nt = J.tilesX * J.tilesY;
+  double w = 0.0, r = 0.0;
+  for (int k = threadIdx.x; k < nt; k += 256) {
+    const double2 p = J.part[k];
+    w = fmax(w, p.x);
+    r += p.y;
+  }
+  const double wb = block_max<256>(w, red);
+  const double rb = block_sum<256>(r, redd);
+  if (threadIdx.x == 0) {
+    s->last[0] = s->cur[0]; s->last[1] = s->cur[1]; s->last[2] = s->cur[2]; s->last[3] = s->cur[3];
+    s->wsnap = (unsigned long long)__double_as_longlong(wb);
+    s->wacc = 0ull;
+    s->steps += 1;
+    s->resLast = sqrt(rb / fmax(s->nGroups, 1.0)) * (double)J.c.resScale;
+    s->resAcc = 0.0;
+    if (J.c.spray) {
+      s->head = (s->head + 4u) % (uint32_t)J.c.NP;
+      s->seed = s->seed + 1u;
     }
   }
 }
@@ -957,6 +1006,10 @@ static hipError_t stage_t(int stage, bool phys, const DevInst* d, int ninst, int
 hipError_t launch_stage(int stage, bool phys, bool fp64, const DevInst* d, int ninst, int ntiles, unsigned* done, hipStream_t s) {
   return fp64 ? stage_t<double, TX64, TY64>(stage, phys, d, ninst, ntiles, done, s)
               : stage_t<float, TX32, TY32>(stage, phys, d, ninst, ntiles, done, s);
+}
+hipError_t launch_finalize(const DevInst* d, int ninst, hipStream_t s) {
+  hipLaunchKernelGGL(k_finalize, dim3(ninst), dim3(256), 0, s, d, ninst);
+  return hipGetLastError();
 }
 hipError_t launch_spray(bool fp64, const DevInst* d, int ninst, int nblk, hipStream_t s) {
   if (fp64) hipLaunchKernelGGL((k_spray<double>), dim3(nblk), dim3(256), 0, s, d, ninst);

@@ -79,10 +79,13 @@ struct DevInst {
   const float4* gg;
   const uint32_t* code;
   const float* lt;
+  const int* exStart;         // per tile (instance-local): first entry in exFaces; tiles + 1 entries
+  const uint2* exFaces;       // faces of merged cells outside their host's tile (x | dir << 31, y)
   float* src;                 // 4 planes of Ns: fuel vapour, ox vapour, x momentum, r momentum
   float* parc;                // 8 planes of NP: x, r, u, v, d, m, type, xi
   const float* tab;           // spray element table 256 x 3 RGBA
   InstState* st;
+  double2* part;              // per tile: (max wave speed, residual sum) of the last stage-2 launch
   Consts c;
 };
 

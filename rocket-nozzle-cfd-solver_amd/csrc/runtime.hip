@@ -43,7 +43,11 @@ struct nz_solver {
   float *ax = nullptr, *ar = nullptr, *lt = nullptr, *jw = nullptr, *src = nullptr, *parc = nullptr, *tab = nullptr;
   float4* gg = nullptr;
   uint32_t* code = nullptr;
+  int* exStart = nullptr;    // per tile: faces of merged cells outside the host's tile
+  uint2* exFaces = nullptr;
+  int exMax = 0;             // most extra faces in one tile
   InstState* st = nullptr;
+  double2* part = nullptr;   // per-tile partials of the stage-2 reductions
   DevInst* dinst = nullptr;
   unsigned* done = nullptr;
   double* dint = nullptr;
@@ -86,7 +90,7 @@ void vfree(void*& p) {
 void free_buffers(nz_solver* s) {
   vfree(s->U0); vfree(s->U1);
   hfree(s->ax); hfree(s->ar); hfree(s->lt); hfree(s->jw); hfree(s->src); hfree(s->parc); hfree(s->tab);
-  hfree(s->gg); hfree(s->code); hfree(s->st); hfree(s->dinst); hfree(s->done); hfree(s->dint);
+  hfree(s->gg); hfree(s->code); hfree(s->exStart); hfree(s->exFaces); hfree(s->st); hfree(s->part); hfree(s->dinst); hfree(s->done); hfree(s->dint);
   hfree(s->wallrow); hfree(s->dstage);
   if (s->hpin) (void)hipHostFree(s->hpin);
   s->hpin = nullptr;
@@ -150,7 +154,8 @@ void base_inst(nz_solver* s) {
   I.Ns = s->Ns;
   I.U0 = s->U0; I.U1 = s->U1;
   I.ax = s->ax; I.ar = s->ar; I.gg = s->gg; I.code = s->code; I.lt = s->lt;
-  I.src = s->src; I.parc = s->parc; I.tab = s->tab; I.st = s->st;
+  I.exStart = s->exStart; I.exFaces = s->exFaces;
+  I.src = s->src; I.parc = s->parc; I.tab = s->tab; I.st = s->st; I.part = s->part;
   I.c.NP = s->NP;
 }
 
@@ -237,11 +242,15 @@ int nz_resize(nz_solver* s, int Nx, int Nr) {
   HIPCHK(hipMalloc(&s->ar, (size_t)(Nr + 1) * Nx * 4), "hipMalloc ar");
   HIPCHK(hipMalloc(&s->gg, (size_t)N * 16), "hipMalloc gg");
   HIPCHK(hipMalloc(&s->code, (size_t)N * 4), "hipMalloc code");
+  HIPCHK(hipMalloc(&s->exStart, ((size_t)s->tilesX * s->tilesY + 1) * sizeof(int)), "hipMalloc extra starts");
+  HIPCHK(hipMemset(s->exStart, 0, ((size_t)s->tilesX * s->tilesY + 1) * sizeof(int)), "memset extra starts");
+  HIPCHK(hipMalloc(&s->exFaces, sizeof(uint2)), "hipMalloc extra faces");
   HIPCHK(hipMalloc(&s->lt, (size_t)N * 4), "hipMalloc lt");
   HIPCHK(hipMalloc(&s->jw, (size_t)Nx * 4 * 4), "hipMalloc jWall");
   HIPCHK(hipMalloc(&s->src, (size_t)s->Ns * 4 * 4), "hipMalloc src");
   HIPCHK(hipMalloc(&s->tab, 256 * 3 * 4 * 4), "hipMalloc tab");
   HIPCHK(hipMalloc(&s->st, sizeof(InstState)), "hipMalloc state info");
+  HIPCHK(hipMalloc(&s->part, (size_t)s->tilesX * s->tilesY * sizeof(double2)), "hipMalloc partials");
   HIPCHK(hipMalloc(&s->dinst, sizeof(DevInst)), "hipMalloc instance");
   HIPCHK(hipMalloc(&s->done, 64), "hipMalloc counter");
   HIPCHK(hipMalloc(&s->dint, 16 * 8), "hipMalloc integrals");
@@ -276,6 +285,58 @@ int nz_get_size(const nz_solver* s, int* Nx, int* Nr) {
   *Nx = s->Nx;
   *Nr = s->Nr;
   return 0;
+}
+
+// For the tile shape in use, every merged cell whose host lies in another tile
+// contributes its three faces that are not faces of the host's tile; the stage
+// kernels evaluate those as extra work items of the host's workgroup.
+static int build_extra_faces(nz_solver* s, const std::vector<uint32_t>& code) {
+  const int Nx = s->Nx, Nr = s->Nr, TX = s->TX, TY = s->TY;
+  const int ntiles = s->tilesX * s->tilesY;
+  std::vector<std::vector<uint2>> lists((size_t)ntiles);
+  auto inTileFace = [&](bool dirR, int fi, int fj, int i0, int j0) {
+    const int fx = fi - i0, fy = fj - j0;
+    return dirR ? (fx >= 0 && fx < TX && fy >= 0 && fy <= TY) : (fx >= 0 && fx <= TX && fy >= 0 && fy < TY);
+  };
+  for (int j = 0; j < Nr; ++j)
+    for (int i = 0; i < Nx; ++i) {
+      const uint32_t c = code[(size_t)j * Nx + i];
+      if ((c & nzk::C_SOLID) || (c & nzk::C_IS_SLAVE)) continue;
+      const uint32_t sl = (c >> nzk::C_SLAVE_SHIFT) & 15u;
+      if (!sl) continue;
+      const int tx = i / TX, ty = j / TY, i0 = tx * TX, j0 = ty * TY;
+      std::vector<uint2>& L = lists[(size_t)ty * s->tilesX + tx];
+      const int di[4] = {-1, 1, 0, 0}, dj[4] = {0, 0, -1, 1};
+      for (int b = 0; b < 4; ++b) {
+        if (!(sl & (1u << b))) continue;
+        const int si = i + di[b], sj = j + dj[b];
+        if (si >= i0 && si < i0 + TX && sj >= j0 && sj < j0 + TY) continue;
+        const bool fdir[4] = {false, false, true, true};
+        const int fi[4] = {si, si + 1, si, si}, fj[4] = {sj, sj, sj, sj + 1};
+        for (int f = 0; f < 4; ++f) {
+          if (inTileFace(fdir[f], fi[f], fj[f], i0, j0)) continue;
+          const uint2 e = make_uint2((uint32_t)fi[f] | (fdir[f] ? 0x80000000u : 0u), (uint32_t)fj[f]);
+          bool dup = false;
+          for (const uint2& q : L) dup |= (q.x == e.x && q.y == e.y);
+          if (!dup) L.push_back(e);
+        }
+      }
+    }
+  std::vector<int> start((size_t)ntiles + 1, 0);
+  std::vector<uint2> faces;
+  s->exMax = 0;
+  for (int t = 0; t < ntiles; ++t) {
+    start[t] = (int)faces.size();
+    faces.insert(faces.end(), lists[t].begin(), lists[t].end());
+    s->exMax = std::max(s->exMax, (int)lists[t].size());
+  }
+  start[ntiles] = (int)faces.size();
+  hfree(s->exFaces);
+  HIPCHK(hipMalloc(&s->exFaces, std::max<size_t>(faces.size(), 1) * sizeof(uint2)), "hipMalloc extra faces");
+  if (!faces.empty()) HIPCHK(hipMemcpy(s->exFaces, faces.data(), faces.size() * sizeof(uint2), hipMemcpyHostToDevice), "upload extra faces");
+  HIPCHK(hipMemcpy(s->exStart, start.data(), start.size() * sizeof(int), hipMemcpyHostToDevice), "upload extra starts");
+  base_inst(s);
+  return send_inst(s);
 }
 
 // convert the reference's texture-shaped geometry into the device layout
@@ -344,7 +405,7 @@ static int upload_geometry(nz_solver* s, const uint8_t* mask, const float* geoA,
   s->nGroups = groups;
   HIPCHK(hipMemcpy(&s->st->nGroups, &s->nGroups, sizeof(double), hipMemcpyHostToDevice), "upload groups");
   std::memcpy(s->mask.data(), mask, (size_t)N);
-  return 0;
+  return build_extra_faces(s, code);
 }
 
 int nz_set_geometry(nz_solver* s, const uint8_t* mask, const float* geoA, const float* geoG, const float* geoM,
@@ -426,6 +487,7 @@ int nz_step(nz_solver* s, int n, const nz_params* P) {
     HIPCHK(nzk::launch_stage(1, false, fp64, s->dinst, 1, ntiles, s->done, s->stream), "stage 1");
     if (spray) HIPCHK(nzk::launch_spray(fp64, s->dinst, 1, s->h.npblk, s->stream), "spray");
     HIPCHK(nzk::launch_stage(2, phys, fp64, s->dinst, 1, ntiles, s->done, s->stream), "stage 2");
+    HIPCHK(nzk::launch_finalize(s->dinst, 1, s->stream), "finalize");
   }
   s->stepCount += n;
   return 0;
@@ -451,6 +513,7 @@ int nz_step_timed(nz_solver* s, int n, const nz_params* P, double* ms) {
     HIPCHK(hipEventRecord(ev[k * 4 + 2], s->stream), "event");
     HIPCHK(nzk::launch_stage(2, phys, fp64, s->dinst, 1, ntiles, s->done, s->stream), "stage 2");
     HIPCHK(hipEventRecord(ev[k * 4 + 3], s->stream), "event");
+    HIPCHK(nzk::launch_finalize(s->dinst, 1, s->stream), "finalize");
   }
   HIPCHK(hipStreamSynchronize(s->stream), "nz_step_timed sync");
   double t[3] = {0, 0, 0};
