@@ -173,11 +173,12 @@ struct StageSmem {
       R Vr[7][TY + 1][TX];
     } vis;
   } u;
-  static constexpr int MAXE = 64;                 // extra faces of merged cells outside the tile
+  static constexpr int MAXE = 192;                // extra faces of merged cells outside the tile
+  static constexpr int MAXF = 64;                 // cells those faces reach beyond the halo
   uint2 ex[MAXE];
-  R Fe[9][MAXE];
-  R Ve[7][MAXE];
-  int nEx;
+  int2 fc[MAXF];
+  R PF[8][MAXF];
+  int nEx, nF;
   double red[16];
   double redd[16];
   int last;
@@ -224,13 +225,21 @@ __global__ void __launch_bounds__(TX* TY) k_stage(const DevInst* __restrict__ in
       sm.P[0][lj][li] = q.r; sm.P[1][lj][li] = q.u; sm.P[2][lj][li] = q.v; sm.P[3][lj][li] = q.p;
       sm.P[4][lj][li] = q.Z; sm.P[5][lj][li] = q.F; sm.P[6][lj][li] = q.G; sm.P[7][lj][li] = q.O;
     }
-    if (tid == 0) {
-      const int e0 = I.exStart[lt_], e1 = I.exStart[lt_ + 1];
-      sm.nEx = min(e1 - e0, S::MAXE);
+    // extra faces of merged cells outside the tile, and the cells they reach beyond
+    // the halo (primitives loaded here in parallel, so no thread waits on them later)
+    const int exBase = I.exStart[lt_];
+    const int nEx = min(I.exStart[lt_ + 1] - exBase, S::MAXE);
+    const int fcBase = I.fcStart[lt_];
+    const int nF = min(I.fcStart[lt_ + 1] - fcBase, S::MAXF);
+    for (int e = tid; e < nEx; e += NT) sm.ex[e] = I.exFaces[exBase + e];
+    for (int e = tid; e < nF; e += NT) {
+      const int2 fcell = I.fcCells[fcBase + e];
+      sm.fc[e] = fcell;
+      const Prim<R> q = prim_at<R>(I, Uin, fcell.x, fcell.y, tsel);
+      sm.PF[0][e] = q.r; sm.PF[1][e] = q.u; sm.PF[2][e] = q.v; sm.PF[3][e] = q.p;
+      sm.PF[4][e] = q.Z; sm.PF[5][e] = q.F; sm.PF[6][e] = q.G; sm.PF[7][e] = q.O;
     }
-    __syncthreads();
-    const int nEx = sm.nEx;
-    for (int e = tid; e < nEx; e += NT) sm.ex[e] = I.exFaces[I.exStart[lt_] + e];
+    R* __restrict__ exFlux = static_cast<R*>(I.exFlux) + (long long)exBase * 16;
     auto P_at = [&](int li, int lj) {
       Prim<R> q;
       q.r = sm.P[0][lj][li]; q.u = sm.P[1][lj][li]; q.v = sm.P[2][lj][li]; q.p = sm.P[3][lj][li];
@@ -243,6 +252,15 @@ __global__ void __launch_bounds__(TX* TY) k_stage(const DevInst* __restrict__ in
     auto anyP = [&](int gi, int gj) {
       const int li = gi - (i0 - 2), lj = gj - (j0 - 2);
       if (li >= 0 && li < S::PW && lj >= 0 && lj < S::PH) return P_at(li, lj);
+      for (int e = 0; e < nF; ++e) {
+        const int2 fcell = sm.fc[e];
+        if (fcell.x == gi && fcell.y == gj) {
+          Prim<R> q;
+          q.r = sm.PF[0][e]; q.u = sm.PF[1][e]; q.v = sm.PF[2][e]; q.p = sm.PF[3][e];
+          q.Z = sm.PF[4][e]; q.F = sm.PF[5][e]; q.G = sm.PF[6][e]; q.O = sm.PF[7][e];
+          return q;
+        }
+      }
       return prim_at<R>(I, Uin, gi, gj, tsel);
     };
     __syncthreads();
@@ -267,8 +285,8 @@ __global__ void __launch_bounds__(TX* TY) k_stage(const DevInst* __restrict__ in
         const int e = f - S::NFX - S::NFR;
         const uint2 ef = sm.ex[e];
         face_flux_w<R>(I, ex_dir(ef), ex_i(ef), ex_j(ef), anyP, F, FY, FO);
-        for (int q = 0; q < 4; ++q) { sm.Fe[q][e] = F[q]; sm.Fe[4 + q][e] = FY[q]; }
-        sm.Fe[8][e] = FO;
+        for (int q = 0; q < 4; ++q) { exFlux[e * 16 + q] = F[q]; exFlux[e * 16 + 4 + q] = FY[q]; }
+        exFlux[e * 16 + 8] = FO;
       }
     }
     __syncthreads();
@@ -287,7 +305,7 @@ __global__ void __launch_bounds__(TX* TY) k_stage(const DevInst* __restrict__ in
       for (int e = 0; e < nEx; ++e) {
         const uint2 ef = sm.ex[e];
         if (ex_dir(ef) == dirR && ex_i(ef) == fi && ex_j(ef) == fj) {
-          for (int q = 0; q < 9; ++q) F9[q] = sm.Fe[q][e];
+          for (int q = 0; q < 9; ++q) F9[q] = exFlux[e * 16 + q];
           return;
         }
       }
@@ -441,8 +459,8 @@ __global__ void __launch_bounds__(TX* TY) k_stage(const DevInst* __restrict__ in
             const int e = f - S::NFX - S::NFR;
             const uint2 ef = sm.ex[e];
             vface_w<R>(I, ex_dir(ef), ex_i(ef), ex_j(ef), anyP, anyT, F, FY);
-            for (int q = 0; q < 4; ++q) sm.Ve[q][e] = F[q];
-            for (int q = 0; q < 3; ++q) sm.Ve[4 + q][e] = FY[q];
+            for (int q = 0; q < 4; ++q) exFlux[e * 16 + 9 + q] = F[q];
+            for (int q = 0; q < 3; ++q) exFlux[e * 16 + 13 + q] = FY[q];
           }
         }
         __syncthreads();
@@ -460,7 +478,7 @@ __global__ void __launch_bounds__(TX* TY) k_stage(const DevInst* __restrict__ in
         for (int e = 0; e < nEx; ++e) {
           const uint2 ef = sm.ex[e];
           if (ex_dir(ef) == dirR && ex_i(ef) == fi && ex_j(ef) == fj) {
-            for (int q = 0; q < 7; ++q) F7[q] = sm.Ve[q][e];
+            for (int q = 0; q < 7; ++q) F7[q] = exFlux[e * 16 + 9 + q];
             return;
           }
         }

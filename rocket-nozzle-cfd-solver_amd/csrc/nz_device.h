@@ -81,6 +81,9 @@ struct DevInst {
   const float* lt;
   const int* exStart;         // per tile (instance-local): first entry in exFaces; tiles + 1 entries
   const uint2* exFaces;       // faces of merged cells outside their host's tile (x | dir << 31, y)
+  const int* fcStart;         // per tile: first entry in fcCells; tiles + 1 entries
+  const int2* fcCells;        // cells those faces reach beyond the tile's 2-cell halo
+  void* exFlux;               // per extra face: 9 inviscid + 7 viscous weighted fluxes (state type)
   float* src;                 // 4 planes of Ns: fuel vapour, ox vapour, x momentum, r momentum
   float* parc;                // 8 planes of NP: x, r, u, v, d, m, type, xi
   const float* tab;           // spray element table 256 x 3 RGBA

@@ -46,6 +46,10 @@ struct nz_solver {
   int* exStart = nullptr;    // per tile: faces of merged cells outside the host's tile
   uint2* exFaces = nullptr;
   int exMax = 0;             // most extra faces in one tile
+  int* fcStart = nullptr;    // per tile: cells those faces reach beyond the 2-cell halo
+  int2* fcCells = nullptr;
+  int fcMax = 0;
+  void* exFlux = nullptr;    // 16 values per extra face (inviscid 9 + viscous 7), state precision
   InstState* st = nullptr;
   double2* part = nullptr;   // per-tile partials of the stage-2 reductions
   DevInst* dinst = nullptr;
@@ -90,7 +94,7 @@ void vfree(void*& p) {
 void free_buffers(nz_solver* s) {
   vfree(s->U0); vfree(s->U1);
   hfree(s->ax); hfree(s->ar); hfree(s->lt); hfree(s->jw); hfree(s->src); hfree(s->parc); hfree(s->tab);
-  hfree(s->gg); hfree(s->code); hfree(s->exStart); hfree(s->exFaces); hfree(s->st); hfree(s->part); hfree(s->dinst); hfree(s->done); hfree(s->dint);
+  hfree(s->gg); hfree(s->code); hfree(s->exStart); hfree(s->exFaces); hfree(s->fcStart); hfree(s->fcCells); vfree(s->exFlux); hfree(s->st); hfree(s->part); hfree(s->dinst); hfree(s->done); hfree(s->dint);
   hfree(s->wallrow); hfree(s->dstage);
   if (s->hpin) (void)hipHostFree(s->hpin);
   s->hpin = nullptr;
@@ -155,6 +159,7 @@ void base_inst(nz_solver* s) {
   I.U0 = s->U0; I.U1 = s->U1;
   I.ax = s->ax; I.ar = s->ar; I.gg = s->gg; I.code = s->code; I.lt = s->lt;
   I.exStart = s->exStart; I.exFaces = s->exFaces;
+  I.fcStart = s->fcStart; I.fcCells = s->fcCells; I.exFlux = s->exFlux;
   I.src = s->src; I.parc = s->parc; I.tab = s->tab; I.st = s->st; I.part = s->part;
   I.c.NP = s->NP;
 }
@@ -245,6 +250,10 @@ int nz_resize(nz_solver* s, int Nx, int Nr) {
   HIPCHK(hipMalloc(&s->exStart, ((size_t)s->tilesX * s->tilesY + 1) * sizeof(int)), "hipMalloc extra starts");
   HIPCHK(hipMemset(s->exStart, 0, ((size_t)s->tilesX * s->tilesY + 1) * sizeof(int)), "memset extra starts");
   HIPCHK(hipMalloc(&s->exFaces, sizeof(uint2)), "hipMalloc extra faces");
+  HIPCHK(hipMalloc(&s->fcStart, ((size_t)s->tilesX * s->tilesY + 1) * sizeof(int)), "hipMalloc far starts");
+  HIPCHK(hipMemset(s->fcStart, 0, ((size_t)s->tilesX * s->tilesY + 1) * sizeof(int)), "memset far starts");
+  HIPCHK(hipMalloc(&s->fcCells, sizeof(int2)), "hipMalloc far cells");
+  HIPCHK(hipMalloc(&s->exFlux, 16 * 8), "hipMalloc extra fluxes");
   HIPCHK(hipMalloc(&s->lt, (size_t)N * 4), "hipMalloc lt");
   HIPCHK(hipMalloc(&s->jw, (size_t)Nx * 4 * 4), "hipMalloc jWall");
   HIPCHK(hipMalloc(&s->src, (size_t)s->Ns * 4 * 4), "hipMalloc src");
@@ -322,19 +331,58 @@ static int build_extra_faces(nz_solver* s, const std::vector<uint32_t>& code) {
         }
       }
     }
-  std::vector<int> start((size_t)ntiles + 1, 0);
+  // cells the extra faces' stencils reach outside the tile's 2-cell halo: inviscid
+  // stencil + tangential pressures, and the 5-point gradient stencils of both cells
+  // of a viscous face when a cell lies outside the 1-cell ring of the gradient tile
+  std::vector<std::vector<int2>> far((size_t)ntiles);
+  for (int t = 0; t < ntiles; ++t) {
+    const int i0 = (t % s->tilesX) * TX, j0 = (t / s->tilesX) * TY;
+    auto inHalo = [&](int i, int j) { return i >= i0 - 2 && i <= i0 + TX + 1 && j >= j0 - 2 && j <= j0 + TY + 1; };
+    auto inRing = [&](int i, int j) { return i >= i0 - 1 && i <= i0 + TX && j >= j0 - 1 && j <= j0 + TY; };
+    std::vector<int2>& F = far[t];
+    auto need = [&](int i, int j) {
+      if (inHalo(i, j)) return;
+      for (const int2& q : F)
+        if (q.x == i && q.y == j) return;
+      F.push_back(make_int2(i, j));
+    };
+    for (const uint2& e : lists[t]) {
+      const bool dirR = (e.x >> 31) != 0;
+      const int fi = (int)(e.x & 0x7fffffffu), fj = (int)e.y;
+      const int dx = dirR ? 0 : 1, dy = dirR ? 1 : 0, tx = dy, ty = dx;
+      need(fi - 2 * dx, fj - 2 * dy); need(fi - dx, fj - dy); need(fi, fj); need(fi + dx, fj + dy);
+      need(fi - dx + tx, fj - dy + ty); need(fi - dx - tx, fj - dy - ty); need(fi + tx, fj + ty); need(fi - tx, fj - ty);
+      const int ci[2] = {dirR ? fi : fi - 1, fi}, cj[2] = {dirR ? fj - 1 : fj, fj};
+      for (int q = 0; q < 2; ++q) {
+        if (inRing(ci[q], cj[q])) continue;
+        need(ci[q], cj[q]); need(ci[q] + 1, cj[q]); need(ci[q] - 1, cj[q]); need(ci[q], cj[q] + 1); need(ci[q], cj[q] - 1);
+      }
+    }
+  }
+  std::vector<int> start((size_t)ntiles + 1, 0), fstart((size_t)ntiles + 1, 0);
   std::vector<uint2> faces;
-  s->exMax = 0;
+  std::vector<int2> cells;
+  s->exMax = s->fcMax = 0;
   for (int t = 0; t < ntiles; ++t) {
     start[t] = (int)faces.size();
     faces.insert(faces.end(), lists[t].begin(), lists[t].end());
     s->exMax = std::max(s->exMax, (int)lists[t].size());
+    fstart[t] = (int)cells.size();
+    cells.insert(cells.end(), far[t].begin(), far[t].end());
+    s->fcMax = std::max(s->fcMax, (int)far[t].size());
   }
   start[ntiles] = (int)faces.size();
+  fstart[ntiles] = (int)cells.size();
   hfree(s->exFaces);
+  hfree(s->fcCells);
+  vfree(s->exFlux);
   HIPCHK(hipMalloc(&s->exFaces, std::max<size_t>(faces.size(), 1) * sizeof(uint2)), "hipMalloc extra faces");
+  HIPCHK(hipMalloc(&s->fcCells, std::max<size_t>(cells.size(), 1) * sizeof(int2)), "hipMalloc far cells");
+  HIPCHK(hipMalloc(&s->exFlux, std::max<size_t>(faces.size(), 1) * 16 * s->rsz), "hipMalloc extra fluxes");
   if (!faces.empty()) HIPCHK(hipMemcpy(s->exFaces, faces.data(), faces.size() * sizeof(uint2), hipMemcpyHostToDevice), "upload extra faces");
+  if (!cells.empty()) HIPCHK(hipMemcpy(s->fcCells, cells.data(), cells.size() * sizeof(int2), hipMemcpyHostToDevice), "upload far cells");
   HIPCHK(hipMemcpy(s->exStart, start.data(), start.size() * sizeof(int), hipMemcpyHostToDevice), "upload extra starts");
+  HIPCHK(hipMemcpy(s->fcStart, fstart.data(), fstart.size() * sizeof(int), hipMemcpyHostToDevice), "upload far starts");
   base_inst(s);
   return send_inst(s);
 }
