@@ -360,12 +360,13 @@ __global__ void __launch_bounds__(TX* TY) k_stage(const DevInst* __restrict__ in
       const R Wx = R(gg.z), Wr = R(gg.w);
       const R wl = sqrt(Wx * Wx + Wr * Wr);
       if (wl > R(1e-6)) {
-        const R pw = wall_pressure<R>(q, q.u * (Wx / wl) + q.v * (Wr / wl));
+        const R iwl = rc(wl);
+        const R pw = wall_pressure<R>(q, q.u * (Wx * iwl) + q.v * (Wr * iwl));
         L[1] -= Wx * pw;
         L[2] -= Wr * pw;
       }
       L[2] += q.p * R(gg.y);
-      const R invV = R(1) / fmax(R(gg.x), R(1e-6));
+      const R invV = rc(fmax(R(gg.x), R(1e-6)));
       const R LB[4] = {LY[0], LY[1], LY[2] - q.G * LY[3], LO};
       R Uq[4], UY[4];
       if (STAGE == 1) {
@@ -380,9 +381,10 @@ __global__ void __launch_bounds__(TX* TY) k_stage(const DevInst* __restrict__ in
         }
       }
       const R rho = fmax(Uq[0], R(1e-7));
-      R vx = Uq[1] / fmax(Uq[0], R(1e-7)), vy = Uq[2] / fmax(Uq[0], R(1e-7));
-      const R sp = sqrt(vx * vx + vy * vy);
-      if (sp > R(c.vMax)) { vx *= R(c.vMax) / sp; vy *= R(c.vMax) / sp; }
+      const R iU0 = rc(fmax(Uq[0], R(1e-7)));
+      R vx = Uq[1] * iU0, vy = Uq[2] * iU0;
+      const R sp = sq(vx * vx + vy * vy);
+      if (sp > R(c.vMax)) { const R f = dv(R(c.vMax), sp); vx *= f; vy *= f; }
       if (!(rho < R(1e30)) || !(vx < R(1e30) && vx > R(-1e30))) {
         guard = true;
         for (int m = 0; m < 4; ++m) { A[m] = R(c.ambCons[m]); B[m] = R(c.ambB[m]); }
@@ -487,7 +489,7 @@ __global__ void __launch_bounds__(TX* TY) k_stage(const DevInst* __restrict__ in
       if (host) {
         const R rho = qS.r, p = qS.p, G = qS.G, velx = qS.u, vely = qS.v;
         const R V = fmax(R(gg.x), R(1e-6));
-        const R Rm = rmix(qS.Z, qS.F, qS.O, c), Tg = p / (rho * Rm);
+        const R Rm = rmix(qS.Z, qS.F, qS.O, c), Tg = dv(p, rho * Rm);
         R L[4] = {0, 0, 0, 0}, LY[3] = {0, 0, 0};
         R T1h[4], T2h[4];
         for (int q = 0; q < 4; ++q) { T1h[q] = sm.u.vis.T[q][ty + 1][tx + 1]; T2h[q] = sm.u.vis.T[4 + q][ty + 1][tx + 1]; }
@@ -502,35 +504,38 @@ __global__ void __launch_bounds__(TX* TY) k_stage(const DevInst* __restrict__ in
             for (int q = 0; q < 3; ++q) LY[q] += -(Fe_[4 + q] - Fw[4 + q]) - (Fn[4 + q] - Fs[4 + q]);
           }
           // axisymmetric hoop stress (solver.js:587-588)
-          const R r = V / fmax(R(gg.y), R(1e-6)), vor = vely / r;
+          const R r = dv(V, fmax(R(gg.y), R(1e-6))), vor = dv(vely, r);
           L[2] -= T2h[0] * (R(2) * vor - R(0.6666667f) * (T1h[0] + T1h[3] + vor)) * R(gg.y);
           // Werner-Wengle wall shear + Reynolds-analogy heat flux (solver.js:590-607)
           const R gx = R(gg.z), gr = R(gg.w);
           const R wl = sqrt(gx * gx + gr * gr);
           if (wl > R(1e-6)) {
-            const R nx = gx / wl, nr = gr / wl;
+            const R iwl = rc(wl);
+            const R nx = gx * iwl, nr = gr * iwl;
             const R vn = velx * nx + vely * nr;
             const R utx = velx - vn * nx, utr = vely - vn * nr;
             const R up = sqrt(utx * utx + utr * utr);
             const R cp = Rm * (G + R(1));
-            const R Taw = Tg + R(0.89) * up * up / (R(2) * cp);
+            const R Taw = Tg + dv(R(0.89) * up * up, R(2) * cp);
             const R Ts = fmax(Tg + R(0.5) * (R(c.Tw) - Tg) + R(0.22) * (Taw - Tg), R(1e-3));
-            const R rs = p / (Rm * Ts), mus = sutherland(Ts * R(c.gas[2]), c);
-            const R dz = R(2) * clampR(R(0.5) * V / wl, R(0.25), R(1));
-            const R nzv = mus / (rs * dz);
-            R tw = up <= R(69.7) * nzv ? R(2) * mus * up / dz
+            const R rs = dv(p, Rm * Ts), mus = sutherland(Ts * R(c.gas[2]), c);
+            const R dz = R(2) * clampR(R(0.5) * V * iwl, R(0.25), R(1));
+            const R nzv = dv(mus, rs * dz);
+            R tw = up <= R(69.7) * nzv ? dv(R(2) * mus * up, dz)
                                        : rs * pow(R(7.187) * pow(nzv, R(1.142857)) + R(0.13769) * pow(nzv, R(0.142857)) * up, R(1.75));
-            tw = fmin(tw, R(0.3) * rho * up * V / fmax(wl * dt, R(1e-9)));
-            if (up > R(1e-6)) { L[1] -= tw * wl * utx / up; L[2] -= tw * wl * utr / up; }
-            const R qw = tw / fmax(up, R(1e-4)) * cp * (Taw - R(c.Tw)) * R(1.24);
-            const R qmax = R(0.3) * p * G * V / fmax(wl * dt, R(1e-9));
+            const R iwdt = rc(fmax(wl * dt, R(1e-9)));
+            tw = fmin(tw, R(0.3) * rho * up * V * iwdt);
+            if (up > R(1e-6)) { const R f = dv(tw * wl, up); L[1] -= f * utx; L[2] -= f * utr; }
+            const R qw = dv(tw, fmax(up, R(1e-4))) * cp * (Taw - R(c.Tw)) * R(1.24);
+            const R qmax = R(0.3) * p * G * V * iwdt;
             L[3] -= clampR(qw, -qmax, qmax) * wl;
           }
           // Zukoski separation in the boundary-layer band (solver.js:613-630)
           if (code & C_BL) {
-            const R up = sqrt(velx * velx + vely * vely), Mn = up / sqrt((R(1) + R(1) / G) * p / rho);
+            const R up = sq(velx * velx + vely * vely), Mn = dv(up, sq(dv((R(1) + rc(G)) * p, rho)));
             if (Mn > R(1)) {
-              const R shx = velx / up, shy = vely / up;
+              const R iup = rc(up);
+              const R shx = velx * iup, shy = vely * iup;
               R pd = R(0);
               for (int n = 4; n <= 6; n += 2)
                 for (int m = 0; m <= 1; ++m) {
@@ -542,7 +547,7 @@ __global__ void __launch_bounds__(TX* TY) k_stage(const DevInst* __restrict__ in
                   if (!(I.code[kd] & C_SOLID)) pd = fmax(pd, load_prim<R>(U1, Ns, kd).p);
                 }
               const R crit = R(1) + R(0.5) * Mn;
-              const R fs = smoothstepR(crit, R(1.2) * crit, pd / p) * V * rho * up;
+              const R fs = smoothstepR(crit, R(1.2) * crit, dv(pd, p)) * V * rho * up;
               L[1] -= fs * velx;
               L[2] -= fs * vely;
             } else if (velx > R(0) && p < R(1.3) * R(c.chemK[3])) {
@@ -551,10 +556,10 @@ __global__ void __launch_bounds__(TX* TY) k_stage(const DevInst* __restrict__ in
             }
           }
         }
-        const R invV = dt / V;
+        const R invV = dv(dt, V);
         const R dO = invV * L[0];
         R Uq[4] = {A[0], A[1] + invV * L[1], A[2] + invV * L[2], A[3] + invV * L[3]};
-        R UY[4] = {B[0] + invV * LY[0], B[1] + invV * LY[1], B[2] + invV * LY[2] / rho, B[3]};
+        R UY[4] = {B[0] + invV * LY[0], B[1] + invV * LY[1], B[2] + dv(invV * LY[2], rho), B[3]};
         if (c.inj) UY[3] += dO;
         // afterburning / injector combustion (solver.js:639-654)
         if (c.chem) {
@@ -572,7 +577,8 @@ __global__ void __launch_bounds__(TX* TY) k_stage(const DevInst* __restrict__ in
           }
         }
         const R rn = fmax(Uq[0], R(1e-7));
-        const R vnx = Uq[1] / rn, vny = Uq[2] / rn;
+        const R irn = rc(rn);
+        const R vnx = Uq[1] * irn, vny = Uq[2] * irn;
         if (!(rn < R(1e30)) || !(vnx < R(1e30) && vnx > R(-1e30))) {
           guard = true;
           for (int m = 0; m < 4; ++m) { A[m] = R(c.ambCons[m]); B[m] = R(c.ambB[m]); }
@@ -598,14 +604,16 @@ __global__ void __launch_bounds__(TX* TY) k_stage(const DevInst* __restrict__ in
       const R dm = s0 + s1;
       if (!(dm == R(0) && s2 == R(0) && s3 == R(0))) {
         src[k] = 0.f; src[Ns + k] = 0.f; src[2 * Ns + k] = 0.f; src[3 * Ns + k] = 0.f;
-        const R invV = R(1) / fmax(R(gg.x), R(1e-6));
-        const R velx = A[1] / fmax(A[0], R(1e-7)), vely = A[2] / fmax(A[0], R(1e-7));
+        const R invV = rc(fmax(R(gg.x), R(1e-6)));
+        const R iA0 = rc(fmax(A[0], R(1e-7)));
+        const R velx = A[1] * iA0, vely = A[2] * iA0;
         const R rn = A[0] + dm * invV;
         const R mx = A[1] + s2 * invV, my = A[2] + s3 * invV;
         R E = A[3] + (s0 * R(c.eF) + s1 * R(c.eO) + (velx * s2 + vely * s3)) * invV;
-        const R phi = fmin(dm * invV / rn, R(1));
+        const R irn = rc(rn);
+        const R phi = fmin(dm * invV * irn, R(1));
         const R Gn = mixR(B[2], R(c.Ge), phi);
-        const R vnx = mx / rn, vny = my / rn;
+        const R vnx = mx * irn, vny = my * irn;
         E = fmax(E, R(0.02) * A[3] + R(0.5) * rn * (vnx * vnx + vny * vny) + R(1e-7) * Gn);
         A[0] = rn; A[1] = mx; A[2] = my; A[3] = E;
         B[0] = B[0] + dm * invV;

@@ -97,11 +97,22 @@ struct Prim {
   R r, u, v, p, Z, F, G, O;
 };
 
+// Division and square root in the state's precision. FP32 uses the hardware
+// reciprocal and square root (v_rcp_f32 / v_sqrt_f32, about 1 ulp; GLSL ES 3.00
+// highp division, which the reference runs on, guarantees only 2.5 ulp); FP64 keeps
+// IEEE division and square root so that the FP64 run is the error reference.
+__device__ __forceinline__ float dv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+__device__ __forceinline__ double dv(double a, double b) { return a / b; }
+__device__ __forceinline__ float rc(float b) { return __builtin_amdgcn_rcpf(b); }
+__device__ __forceinline__ double rc(double b) { return 1.0 / b; }
+__device__ __forceinline__ float sq(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ double sq(double x) { return sqrt(x); }
+
 template <typename R>
 __device__ __forceinline__ R clampR(R x, R lo, R hi) { return fmin(fmax(x, lo), hi); }
 template <typename R>
 __device__ __forceinline__ R smoothstepR(R e0, R e1, R x) {
-  const R t = clampR<R>((x - e0) / (e1 - e0), R(0), R(1));
+  const R t = clampR<R>(dv(x - e0, e1 - e0), R(0), R(1));
   return t * t * (R(3) - R(2) * t);
 }
 template <typename R>
@@ -112,17 +123,18 @@ template <typename R>
 __device__ __forceinline__ Prim<R> to_prim(R a0, R a1, R a2, R a3, R b0, R b1, R b2, R b3) {
   Prim<R> q;
   q.r = fmax(a0, R(1e-7));
-  q.u = a1 / q.r;
-  q.v = a2 / q.r;
+  const R ir = rc(q.r);
+  q.u = a1 * ir;
+  q.v = a2 * ir;
   q.G = fmax(b2, R(0.5));
-  q.p = fmax((a3 - R(0.5) * q.r * (q.u * q.u + q.v * q.v)) / q.G, R(1e-7));
-  q.Z = clampR(b0 / q.r, R(0), R(1));
-  q.F = clampR(b1 / q.r, R(0), R(1));
-  q.O = clampR(b3 / q.r, R(0), R(1));
+  q.p = fmax(dv(a3 - R(0.5) * q.r * (q.u * q.u + q.v * q.v), q.G), R(1e-7));
+  q.Z = clampR(b0 * ir, R(0), R(1));
+  q.F = clampR(b1 * ir, R(0), R(1));
+  q.O = clampR(b3 * ir, R(0), R(1));
   return q;
 }
 template <typename R>
-__device__ __forceinline__ R sound_speed(const Prim<R>& q) { return sqrt((R(1) + R(1) / q.G) * q.p / q.r); }
+__device__ __forceinline__ R sound_speed(const Prim<R>& q) { return sq(dv((R(1) + rc(q.G)) * q.p, q.r)); }
 
 template <typename R>
 __device__ __forceinline__ Prim<R> wall_prim() {
@@ -202,10 +214,10 @@ template <typename R>
 __device__ __forceinline__ R van_leer(R a, R b) {
   R den = a + b;
   if (fabs(den) < R(1e-30)) den = R(1);
-  return R(2) * fmax(a * b, R(0)) / den;
+  return dv(R(2) * fmax(a * b, R(0)), den);
 }
 template <typename R>
-__device__ __forceinline__ R jump_ratio(R a, R b) { return fabs(a - b) / fmin(a, b); }
+__device__ __forceinline__ R jump_ratio(R a, R b) { return dv(fabs(a - b), fmin(a, b)); }
 
 // Face flux: MUSCL van Leer reconstruction with positivity fallback, wall mirror
 // states, HLLC with Roe-averaged wave speeds blended to HLL by `hllw`
@@ -250,16 +262,17 @@ __device__ __forceinline__ void face_flux(Prim<R> c0, Prim<R> c1, Prim<R> c2, Pr
   const R GL = WL.G, GR = WR.G;
   const R rL = WL.r, uL = WL.u, vL = WL.v, pL = WL.p;
   const R rR = WR.r, uR = WR.u, vR = WR.v, pR = WR.p;
-  const R cL = sqrt((R(1) + R(1) / GL) * pL / rL), cR = sqrt((R(1) + R(1) / GR) * pR / rR);
+  const R irL = rc(rL), irR = rc(rR);
+  const R cL = sq((R(1) + rc(GL)) * pL * irL), cR = sq((R(1) + rc(GR)) * pR * irR);
   const R EL = pL * GL + R(0.5) * rL * (uL * uL + vL * vL);
   const R ER = pR * GR + R(0.5) * rR * (uR * uR + vR * vR);
-  const R sL = sqrt(rL), sR = sqrt(rR);
-  const R isum = R(1) / (sL + sR);
+  const R sL = sq(rL), sR = sq(rR);
+  const R isum = rc(sL + sR);
   const R uT = (sL * uL + sR * uR) * isum;
-  const R HT = (sL * (EL + pL) / rL + sR * (ER + pR) / rR) * isum;
+  const R HT = (sL * (EL + pL) * irL + sR * (ER + pR) * irR) * isum;
   const R vT = (sL * vL + sR * vR) * isum;
   const R GT = (sL * GL + sR * GR) * isum;
-  const R cT = sqrt(fmax((HT - R(0.5) * (uT * uT + vT * vT)) / GT, R(1e-12)));
+  const R cT = sq(fmax(dv(HT - R(0.5) * (uT * uT + vT * vT), GT), R(1e-12)));
   const R SL = fmin(uL - cL, uT - cT);
   const R SR = fmax(uR + cR, uT + cT);
   const R mL = rL * uL, mR = rR * uR;
@@ -277,7 +290,7 @@ __device__ __forceinline__ void face_flux(Prim<R> c0, Prim<R> c1, Prim<R> c2, Pr
     FY[2] = uR * GR; FY[3] = uR;
     return;
   }
-  const R inv = R(1) / (SR - SL);
+  const R inv = rc(SR - SL);
   const R SLR = SL * SR;
   const R Fh0 = (SR * FL0 - SL * FR0 + SLR * (rR - rL)) * inv;
   const R Fh1 = (SR * FL1 - SL * FR1 + SLR * (rR * uR - rL * uL)) * inv;
@@ -286,15 +299,17 @@ __device__ __forceinline__ void face_flux(Prim<R> c0, Prim<R> c1, Prim<R> c2, Pr
   const R Yh0 = (SR * mL * WL.Z - SL * mR * WR.Z + SLR * (rR * WR.Z - rL * WL.Z)) * inv;
   const R Yh1 = (SR * mL * WL.F - SL * mR * WR.F + SLR * (rR * WR.F - rL * WL.F)) * inv;
   const R Yh2 = (SR * mL * WL.O - SL * mR * WR.O + SLR * (rR * WR.O - rL * WL.O)) * inv;
-  const R Ss = (pR - pL + mL * (SL - uL) - mR * (SR - uR)) / (rL * (SL - uL) - rR * (SR - uR));
+  const R Ss = dv(pR - pL + mL * (SL - uL) - mR * (SR - uR), rL * (SL - uL) - rR * (SR - uR));
   const bool left = Ss >= R(0);
   const R rK = left ? rL : rR, uK = left ? uL : uR, vK = left ? vL : vR, pK = left ? pL : pR;
   const R EK = left ? EL : ER, SK = left ? SL : SR;
   const R YK0 = left ? WL.Z : WR.Z, YK1 = left ? WL.F : WR.F, YK2 = left ? WL.O : WR.O;
   const R FK0 = left ? FL0 : FR0, FK1 = left ? FL1 : FR1, FK2 = left ? FL2 : FR2, FK3 = left ? FL3 : FR3;
   const R mK = rK * uK;
-  const R kk = rK * (SK - uK) / (SK - Ss);
-  const R Us0 = kk, Us1 = kk * Ss, Us2 = kk * vK, Us3 = kk * (EK / rK + (Ss - uK) * (Ss + pK / (rK * (SK - uK))));
+  const R iSKs = rc(SK - Ss);
+  const R kk = rK * (SK - uK) * iSKs;
+  const R irK = rc(rK);
+  const R Us0 = kk, Us1 = kk * Ss, Us2 = kk * vK, Us3 = kk * (EK * irK + (Ss - uK) * (Ss + dv(pK * irK, SK - uK)));
   F[0] = mixR(FK0 + SK * (Us0 - rK), Fh0, hllw);
   F[1] = mixR(FK1 + SK * (Us1 - mK), Fh1, hllw);
   F[2] = mixR(FK2 + SK * (Us2 - rK * vK), Fh2, hllw);
@@ -302,7 +317,7 @@ __device__ __forceinline__ void face_flux(Prim<R> c0, Prim<R> c1, Prim<R> c2, Pr
   FY[0] = mixR(mK * YK0 + SK * (kk * YK0 - rK * YK0), Yh0, hllw);
   FY[1] = mixR(mK * YK1 + SK * (kk * YK1 - rK * YK1), Yh1, hllw);
   FO = mixR(mK * YK2 + SK * (kk * YK2 - rK * YK2), Yh2, hllw);
-  const R uf = uK + SK * ((SK - uK) / (SK - Ss) - R(1));
+  const R uf = uK + SK * ((SK - uK) * iSKs - R(1));
   FY[2] = uf * (left ? GL : GR);
   FY[3] = uf;
 }
@@ -318,18 +333,18 @@ __device__ __forceinline__ R hll_weight(R p1, R p2, R p1a, R p1b, R p2a, R p2b, 
 // reflected-Riemann pressure on a slip wall (solver.js:301-306)
 template <typename R>
 __device__ __forceinline__ R wall_pressure(const Prim<R>& q, R un) {
-  const R g = R(1) + R(1) / q.G, p = q.p, rho = q.r, c = sqrt(g * p / rho);
+  const R g = R(1) + rc(q.G), p = q.p, rho = q.r, c = sq(dv(g * p, rho));
   if (un > R(0)) {
     const R a = R(0.25) * (g + R(1)) * un;
-    return p + rho * un * (a + sqrt(a * a + c * c));
+    return p + rho * un * (a + sq(a * a + c * c));
   }
-  const R f = fmax(R(1) + R(0.5) * (g - R(1)) * un / c, R(0));
-  return p * pow(f, R(2) * g / (g - R(1)));
+  const R f = fmax(R(1) + dv(R(0.5) * (g - R(1)) * un, c), R(0));
+  return p * pow(f, dv(R(2) * g, g - R(1)));
 }
 
 // Sutherland viscosity and mixture gas constant (solver.js:448-449)
 template <typename R>
-__device__ __forceinline__ R sutherland(R TK, const Consts& c) { return R(c.gas[3]) * R(1.458e-6) * TK * sqrt(TK) / (TK + R(110.4)); }
+__device__ __forceinline__ R sutherland(R TK, const Consts& c) { return dv(R(c.gas[3]) * R(1.458e-6) * TK * sq(TK), TK + R(110.4)); }
 template <typename R>
 __device__ __forceinline__ R rmix(R Z, R F, R O, const Consts& c) {
   return (R(1) - Z) * R(c.gas[0]) + F * R(c.Rs[0]) + O * R(c.Rs[1]) + fmax(Z - F - O, R(0)) * R(c.Rs[2]);
@@ -342,24 +357,25 @@ __device__ __forceinline__ void vcell(const Prim<R>& w0, const Prim<R>& wE, cons
                                       const Prim<R>& wS, int i, int j, R l, R dt, const Consts& c, R* T1, R* T2) {
   const R fE = wE.r < R(0) ? R(0) : R(1), fW = wW.r < R(0) ? R(0) : R(1);
   const R fN = wN.r < R(0) ? R(0) : R(1), fS = wS.r < R(0) ? R(0) : R(1);
-  const R dxn = fmax(fE + fW, R(1)), drn = fmax(fN + fS, R(1));
-  const R ux = (fE * (wE.u - w0.u) + fW * (w0.u - wW.u)) / dxn;
-  const R vx = (fE * (wE.v - w0.v) + fW * (w0.v - wW.v)) / dxn;
-  const R ur = (fN * (wN.u - w0.u) + fS * (w0.u - wS.u)) / drn;
-  const R vr = (fN * (wN.v - w0.v) + fS * (w0.v - wS.v)) / drn;
-  const R r = R(j) + R(0.5), vor = w0.v / r;
+  const R idx = rc(fmax(fE + fW, R(1))), idr = rc(fmax(fN + fS, R(1)));
+  const R ux = (fE * (wE.u - w0.u) + fW * (w0.u - wW.u)) * idx;
+  const R vx = (fE * (wE.v - w0.v) + fW * (w0.v - wW.v)) * idx;
+  const R ur = (fN * (wN.u - w0.u) + fS * (w0.u - wS.u)) * idr;
+  const R vr = (fN * (wN.v - w0.v) + fS * (w0.v - wS.v)) * idr;
+  const R r = R(j) + R(0.5), vor = dv(w0.v, r);
   const R div = ux + vr + vor;
   const R S2 = R(2) * (ux * ux + vr * vr + vor * vor) + (ur + vx) * (ur + vx) - R(0.6666667f) * div * div;
   const R om = vx - ur;
-  const R sens = om * om / (om * om + div * div + R(1e-8));
-  const R Smag = sqrt(fmax(S2, R(0))) * sens;
+  const R sens = dv(om * om, om * om + div * div + R(1e-8));
+  const R Smag = sq(fmax(S2, R(0))) * sens;
   const R Rm = rmix(w0.Z, w0.F, w0.O, c);
-  const R mu = sutherland(w0.p / (w0.r * Rm) * R(c.gas[2]), c);
+  const R ir0 = rc(w0.r);
+  const R mu = sutherland(dv(w0.p * ir0, Rm) * R(c.gas[2]), c);
   R mut = w0.r * l * l * Smag;
   if (R(i) < R(c.chamberNu[1])) mut = fmax(mut, w0.r * R(c.chamberNu[0]));
   T1[0] = ux; T1[1] = ur; T1[2] = vx; T1[3] = vr;
-  T2[0] = fmin(mu + mut, R(0.08) * w0.r / fmax(dt, R(1e-9)));
-  T2[1] = (w0.G + R(1)) * w0.p / w0.r;
+  T2[0] = fmin(mu + mut, dv(R(0.08) * w0.r, fmax(dt, R(1e-9))));
+  T2[1] = (w0.G + R(1)) * w0.p * ir0;
   T2[2] = Smag;
   T2[3] = mu;
 }
@@ -382,7 +398,7 @@ __device__ __forceinline__ void visc_face(bool dirR, int j, const Prim<R>& w1, c
     ux = w2.u - w1.u; vx = w2.v - w1.v;
     ur = R(0.5) * (t1[1] + t2[1]); vr = R(0.5) * (t1[3] + t2[3]);
   }
-  const R div = ux + vr + vf / r;
+  const R div = ux + vr + dv(vf, r);
   const R txr = mu * (ur + vx);
   const R k = mu * PR_INV;
   F[0] = -k * (w2.O - w1.O);
