@@ -174,6 +174,71 @@ int nz_set_probes(nz_solver* s, const nz_probes* pr);
 int nz_integrals_read(nz_solver* s, const nz_params* P, nz_integrals* out);
 
 /* ------------------------------------------------------------------------ */
+/* Batched solves with continuous admission. No counterpart in the          */
+/* reference, which runs one GPUSolver per page (app.js:151-167); for       */
+/* callers that advance many independent solves (parameter sweeps, a        */
+/* request loop). Every resident solve of any grid size advances in the     */
+/* same launches (one per kernel per step over all residents' tiles), from  */
+/* the same kernels as nz_step, so a solve's result does not depend on what */
+/* else is resident (bit-identical without spray; the spray's float-atomic  */
+/* deposit order is not deterministic in either path). Jobs are admitted    */
+/* first-fit in submission order while the device-memory budget and the     */
+/* resident limit allow; a solve retires when it meets its stop test, its   */
+/* memory is freed and queued jobs take its place at the next run call.     */
+/* ------------------------------------------------------------------------ */
+typedef struct nz_batch nz_batch;
+typedef struct nz_batch_job {
+  int32_t Nx, Nr;
+  const uint8_t* mask;                     /* as nz_set_geometry */
+  const float *geoA, *geoG, *geoM, *jWall; /* jWall may be NULL */
+  const float *A, *B;                      /* initial state, as nz_set_state */
+  nz_params P;                             /* parameters, held for the whole solve */
+  const float* injector;                   /* NULL, or 256*3*4 floats (nz_spray_set_injector) */
+  int32_t parcels;                         /* parcel count when an injector is given */
+  nz_probes probes;                        /* dx > 0: integrated quantities at retirement */
+  double t_end;                            /* stop at this simulated grid time (<= 0: off) */
+  double res_tol;                          /* stop at residual <= res_tol (<= 0: off) ... */
+  int32_t min_steps;                       /* ... once this many steps are done */
+  int32_t max_steps;                       /* stop after this many steps (<= 0: off) */
+  int32_t keep_state;                      /* keep the final state for nz_batch_read_state */
+} nz_batch_job;
+enum { NZ_JOB_QUEUED = 0, NZ_JOB_RUNNING = 1, NZ_JOB_DONE = 2 };
+enum { NZ_STOP_NONE = 0, NZ_STOP_RESIDUAL = 1, NZ_STOP_TIME = 2, NZ_STOP_STEPS = 3 };
+typedef struct nz_batch_result {
+  int32_t state, stop;                     /* NZ_JOB_*, NZ_STOP_* */
+  int64_t steps;
+  double t_grid, dt, residual;              /* at the last completed step */
+  double t_submit, t_admit, t_done;        /* host steady clock, s */
+  int32_t have_integrals;
+  nz_integrals integ;                      /* at retirement, when probes were given */
+} nz_batch_result;
+/* precision 32 or 64; hbm_budget in bytes for all residents' buffers;
+ * max_resident >= 1 */
+int nz_batch_create(nz_batch** out, int precision, uint64_t hbm_budget, int max_resident);
+int nz_batch_destroy(nz_batch* b);
+/* device bytes one solve of this size holds while resident */
+uint64_t nz_batch_instance_bytes(int Nx, int Nr, int precision, int parcels);
+/* copy a job into the queue; *id identifies it from then on. Fails if the job
+ * alone exceeds the budget. */
+int nz_batch_submit(nz_batch* b, const nz_batch_job* job, int64_t* id);
+/* admit what fits, advance every resident `steps` steps (a captured graph
+ * when enabled), then retire the solves that stopped. Returns after the
+ * device has finished; *resident / *queued (may be NULL) count what is left. */
+int nz_batch_run(nz_batch* b, int steps, int* resident, int* queued);
+int nz_batch_result_get(nz_batch* b, int64_t id, nz_batch_result* out);
+/* state of a running solve, or of a finished one submitted with keep_state */
+int nz_batch_read_state(nz_batch* b, int64_t id, float* A, float* B);
+/* forget a finished job (its kept state and result) */
+int nz_batch_release(nz_batch* b, int64_t id);
+/* 1: capture each run's launches in a HIP graph and replay it while the
+ * resident set and step count are unchanged (default 1) */
+int nz_batch_set_graph(nz_batch* b, int on);
+int nz_batch_memory(const nz_batch* b, uint64_t* used, uint64_t* budget);
+/* ids of the jobs that finished in the last nz_batch_run; returns the count
+ * (at most cap are written) */
+int nz_batch_finished(const nz_batch* b, int64_t* ids, int cap);
+
+/* ------------------------------------------------------------------------ */
 /* Host setup: buildSimSetup / computeParams / buildCutGeometry /            */
 /* buildInjectorTable restated in C++ (simsetup.js:7-311, cutcell.js:14-186, */
 /* physics.js:7-199). Used to build solver inputs without the browser.       */

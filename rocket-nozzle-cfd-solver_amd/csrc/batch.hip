@@ -1,0 +1,440 @@
+// batch.hip — batched solves with continuous admission (nz_batch_*, include/nozzle.h).
+//
+// Resident solves of any grid size share every launch: the stage kernels run one
+// workgroup per tile over the concatenated tile ranges of all residents (each
+// workgroup finds its instance by binary search over the descriptor table), the
+// spray kernel over their concatenated parcel blocks, the finalize kernel one
+// workgroup per resident. Per-instance stop tests (simulated time, residual, step
+// count) run on the device in the finalize kernel; a stopped instance's workgroups
+// return at once, so nothing waits on the host between steps. Between run calls the
+// host retires stopped solves (freeing their buffers) and admits queued jobs
+// first-fit by device-memory budget. The step sequence of a run is captured in a
+// HIP graph and replayed while the resident set and step count stay the same.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "nz_inst.h"
+#include "nz_launch.h"
+
+using nzk::Consts;
+using nzk::DevInst;
+using nzk::InstState;
+
+namespace {
+
+double now_s() {
+  using namespace std::chrono;
+  return duration<double>(steady_clock::now().time_since_epoch()).count();
+}
+
+// a submitted job: the caller's arrays copied, until it is admitted
+struct Job {
+  int64_t id = 0;
+  int Nx = 0, Nr = 0;
+  std::vector<uint8_t> mask;
+  std::vector<float> geoA, geoG, geoM, jWall, A, B, tab;
+  nz_params P{};
+  int parcels = 0;
+  nz_probes probes{};
+  double tEnd = 0, resTol = 0;
+  int minSteps = 0, maxSteps = 0;
+  bool keep = false;
+  size_t bytes = 0;
+  double tSubmit = 0;
+};
+
+struct Resident {
+  std::unique_ptr<Job> job;  // inputs are dropped after admission; metadata kept
+  nz::Inst I;
+  int slot = 0;              // index into the batch's state-info array
+  double tAdmit = 0;
+};
+
+struct Done {
+  nz_batch_result r{};
+  std::vector<float> A, B;   // kept final state
+};
+
+}  // namespace
+
+struct nz_batch {
+  int prec = 32;
+  uint64_t budget = 0, used = 0;
+  int maxRes = 1;
+  hipStream_t stream = nullptr;
+  InstState* st = nullptr;       // device, one per slot
+  InstState* stHost = nullptr;   // pinned copy read after each run
+  DevInst* dtab = nullptr;       // device descriptor table (resident order)
+  DevInst* htab = nullptr;       // pinned host copy for the upload
+  std::vector<int> freeSlots;
+  std::vector<std::unique_ptr<Resident>> res;
+  std::deque<std::unique_ptr<Job>> queue;
+  std::map<int64_t, Done> done;
+  std::vector<int64_t> finished;
+  int64_t nextId = 1;
+  bool tableDirty = true;
+  int ntiles = 0, npblk = 0;
+  bool phys = false;
+  bool useGraph = true;
+  hipGraphExec_t graph = nullptr;
+  int graphSteps = 0;
+};
+
+namespace {
+
+void drop_graph(nz_batch* b) {
+  if (b->graph) (void)hipGraphExecDestroy(b->graph);
+  b->graph = nullptr;
+  b->graphSteps = 0;
+}
+
+int admit(nz_batch* b, std::unique_ptr<Job> job) {
+  auto r = std::make_unique<Resident>();
+  r->slot = b->freeSlots.back();
+  nz::Inst& I = r->I;
+  I.NP = job->tab.empty() ? 4 : job->parcels;
+  // a fresh solve: the slot's state info starts zeroed with spray seed 1 (spray.js:221)
+  if (int rc = nz::inst_alloc(I, job->Nx, job->Nr, b->prec, b->st + r->slot)) { nz::inst_free(I); return rc; }
+  int rc = nz::inst_set_geometry(I, job->mask.data(), job->geoA.data(), job->geoG.data(), job->geoM.data(),
+                                 job->jWall.empty() ? nullptr : job->jWall.data());
+  if (!rc && !job->tab.empty()) rc = nz::inst_set_injector(I, job->tab.data());
+  if (!rc) rc = nz::inst_spray_alloc(I, I.NP, b->stream);
+  if (!rc) {
+    nz::fill_consts(I, job->P, I.c, job->tEnd, job->resTol, job->minSteps, job->maxSteps);
+    rc = nz::inst_set_state(I, job->A.data(), job->B.data(), b->stream);
+  }
+  if (rc) { nz::inst_free(I); return rc; }
+  b->freeSlots.pop_back();
+  b->used += job->bytes;
+  // the inputs are on the device now
+  job->mask.clear(); job->mask.shrink_to_fit();
+  for (auto* v : {&job->geoA, &job->geoG, &job->geoM, &job->jWall, &job->A, &job->B, &job->tab}) {
+    v->clear();
+    v->shrink_to_fit();
+  }
+  r->tAdmit = now_s();
+  r->job = std::move(job);
+  b->res.push_back(std::move(r));
+  b->tableDirty = true;
+  return 0;
+}
+
+int admit_queued(nz_batch* b) {
+  for (auto it = b->queue.begin(); it != b->queue.end();) {
+    if ((int)b->res.size() >= b->maxRes) break;
+    if (b->used + (*it)->bytes > b->budget) { ++it; continue; }
+    std::unique_ptr<Job> j = std::move(*it);
+    it = b->queue.erase(it);
+    if (int rc = admit(b, std::move(j))) return rc;
+  }
+  return 0;
+}
+
+int upload_table(nz_batch* b) {
+  int t = 0, pb = 0;
+  b->phys = false;
+  for (size_t n = 0; n < b->res.size(); ++n) {
+    nz::Inst& I = b->res[n]->I;
+    DevInst d = nz::inst_desc(I);
+    d.c = I.c;
+    d.c.NP = I.NP;
+    d.tile0 = t;
+    d.pblk0 = pb;
+    d.npblk = d.c.spray ? (I.NP + 255) / 256 : 0;
+    t += I.tilesX * I.tilesY;
+    pb += d.npblk;
+    b->phys |= (d.c.visc || d.c.chem);
+    b->htab[n] = d;
+  }
+  b->ntiles = t;
+  b->npblk = pb;
+  if (!b->res.empty())
+    HIPCHK(hipMemcpyAsync(b->dtab, b->htab, b->res.size() * sizeof(DevInst), hipMemcpyHostToDevice, b->stream), "batch table");
+  HIPCHK(hipStreamSynchronize(b->stream), "batch table sync");
+  b->tableDirty = false;
+  drop_graph(b);
+  return 0;
+}
+
+int enqueue_steps(nz_batch* b, int steps) {
+  const bool fp64 = b->prec == 64;
+  const int n = (int)b->res.size();
+  for (int k = 0; k < steps; ++k) {
+    HIPCHK(nzk::launch_stage(1, false, fp64, b->dtab, n, b->ntiles, b->stream), "batch stage 1");
+    if (b->npblk) HIPCHK(nzk::launch_spray(fp64, b->dtab, n, b->npblk, b->stream), "batch spray");
+    HIPCHK(nzk::launch_stage(2, b->phys, fp64, b->dtab, n, b->ntiles, b->stream), "batch stage 2");
+    HIPCHK(nzk::launch_finalize(b->dtab, n, b->stream), "batch finalize");
+  }
+  return 0;
+}
+
+int run_steps(nz_batch* b, int steps) {
+  if (!b->useGraph) return enqueue_steps(b, steps);
+  if (!b->graph || b->graphSteps != steps) {
+    drop_graph(b);
+    HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal), "batch capture begin");
+    const int rc = enqueue_steps(b, steps);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(b->stream, &g);
+    if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
+    if (e != hipSuccess) return nz_fail(-10, std::string("batch capture end: ") + hipGetErrorString(e));
+    const hipError_t ei = hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ei != hipSuccess) { b->graph = nullptr; return nz_fail(-10, std::string("batch graph: ") + hipGetErrorString(ei)); }
+    b->graphSteps = steps;
+  }
+  HIPCHK(hipGraphLaunch(b->graph, b->stream), "batch graph launch");
+  return 0;
+}
+
+int retire(nz_batch* b, size_t n, const InstState& is) {
+  Resident& r = *b->res[n];
+  Done d;
+  nz_batch_result& o = d.r;
+  o.state = NZ_JOB_DONE;
+  o.stop = (is.flags & 2u) ? NZ_STOP_RESIDUAL : (is.flags & 4u) ? NZ_STOP_TIME : (is.flags & 8u) ? NZ_STOP_STEPS : NZ_STOP_NONE;
+  o.steps = is.steps;
+  o.t_grid = is.last[2];
+  o.dt = is.last[0];
+  o.residual = is.resLast;
+  o.t_submit = r.job->tSubmit;
+  o.t_admit = r.tAdmit;
+  o.t_done = now_s();
+  if (r.job->probes.dx > 0) {
+    DevInst desc = nz::inst_desc(r.I);
+    desc.c = r.I.c;
+    if (int rc = nz::inst_integrals(r.I, desc, r.job->probes, b->stream, &o.integ)) return rc;
+    o.have_integrals = 1;
+  }
+  if (r.job->keep) {
+    const long long N = (long long)r.I.Nx * r.I.Nr;
+    d.A.resize((size_t)N * 4);
+    d.B.resize((size_t)N * 4);
+    float* tmp = nullptr;
+    HIPCHK(hipMalloc(&tmp, (size_t)N * 32), "batch keep staging");
+    const hipError_t e1 = nzk::launch_pack(b->prec == 64, r.I.U0, r.I.Ns, r.I.Nx, 0, 0, 0, r.I.Nx, r.I.Nr, tmp, b->stream);
+    const hipError_t e2 = nzk::launch_pack(b->prec == 64, r.I.U0, r.I.Ns, r.I.Nx, 4, 0, 0, r.I.Nx, r.I.Nr, tmp + N * 4, b->stream);
+    hipError_t e3 = hipSuccess, e4 = hipSuccess;
+    if (e1 == hipSuccess && e2 == hipSuccess) {
+      e3 = hipMemcpyAsync(d.A.data(), tmp, (size_t)N * 16, hipMemcpyDeviceToHost, b->stream);
+      if (e3 == hipSuccess) e3 = hipMemcpyAsync(d.B.data(), tmp + N * 4, (size_t)N * 16, hipMemcpyDeviceToHost, b->stream);
+      e4 = hipStreamSynchronize(b->stream);
+    }
+    (void)hipFree(tmp);
+    if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess || e4 != hipSuccess) return nz_fail(-10, "batch: keep state failed");
+  }
+  b->used -= r.job->bytes;
+  b->freeSlots.push_back(r.slot);
+  b->finished.push_back(r.job->id);
+  b->done[r.job->id] = std::move(d);
+  nz::inst_free(r.I);
+  b->res.erase(b->res.begin() + (long)n);
+  b->tableDirty = true;
+  return 0;
+}
+
+Resident* find_res(nz_batch* b, int64_t id) {
+  for (auto& r : b->res)
+    if (r->job->id == id) return r.get();
+  return nullptr;
+}
+
+}  // namespace
+
+extern "C" {
+
+uint64_t nz_batch_instance_bytes(int Nx, int Nr, int precision, int parcels) {
+  return (uint64_t)nz::inst_bytes(Nx, Nr, precision, parcels > 0 ? parcels : 4);
+}
+
+int nz_batch_create(nz_batch** out, int precision, uint64_t hbm_budget, int max_resident) {
+  if (!out) return nz_fail(-1, "nz_batch_create: null output");
+  *out = nullptr;
+  if (precision != 32 && precision != 64) return nz_fail(-1, "nz_batch_create: precision must be 32 or 64");
+  if (max_resident < 1 || max_resident > 4096) return nz_fail(-1, "nz_batch_create: max_resident must be in [1, 4096]");
+  if (hbm_budget == 0) return nz_fail(-1, "nz_batch_create: zero memory budget");
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return nz_fail(-5, "nz_batch_create: no HIP device available");
+  auto b = std::make_unique<nz_batch>();
+  b->prec = precision;
+  b->budget = hbm_budget;
+  b->maxRes = max_resident;
+  HIPCHK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking), "nz_batch_create: stream");
+  auto fail = [&](const char* what) {
+    if (b->st) (void)hipFree(b->st);
+    if (b->dtab) (void)hipFree(b->dtab);
+    if (b->stHost) (void)hipHostFree(b->stHost);
+    if (b->htab) (void)hipHostFree(b->htab);
+    (void)hipStreamDestroy(b->stream);
+    return nz_fail(-10, what);
+  };
+  if (hipMalloc(&b->st, sizeof(InstState) * max_resident) != hipSuccess) return fail("nz_batch_create: state info");
+  if (hipMalloc(&b->dtab, sizeof(DevInst) * max_resident) != hipSuccess) return fail("nz_batch_create: table");
+  if (hipHostMalloc(&b->stHost, sizeof(InstState) * max_resident, hipHostMallocDefault) != hipSuccess) return fail("nz_batch_create: pinned state");
+  if (hipHostMalloc(&b->htab, sizeof(DevInst) * max_resident, hipHostMallocDefault) != hipSuccess) return fail("nz_batch_create: pinned table");
+  for (int k = max_resident - 1; k >= 0; --k) b->freeSlots.push_back(k);
+  *out = b.release();
+  return 0;
+}
+
+int nz_batch_destroy(nz_batch* b) {
+  if (!b) return 0;
+  (void)hipStreamSynchronize(b->stream);
+  drop_graph(b);
+  for (auto& r : b->res) nz::inst_free(r->I);
+  b->res.clear();
+  (void)hipFree(b->st);
+  (void)hipFree(b->dtab);
+  (void)hipHostFree(b->stHost);
+  (void)hipHostFree(b->htab);
+  (void)hipStreamDestroy(b->stream);
+  delete b;
+  return 0;
+}
+
+int nz_batch_submit(nz_batch* b, const nz_batch_job* jb, int64_t* id) {
+  if (!b || !jb || !id) return nz_fail(-1, "nz_batch_submit: null argument");
+  if (jb->Nx < 2 || jb->Nr < 2 || (long long)jb->Nx * jb->Nr > (1LL << 28)) return nz_fail(-1, "nz_batch_submit: grid size out of range");
+  if (!jb->mask || !jb->geoA || !jb->geoG || !jb->geoM || !jb->A || !jb->B) return nz_fail(-1, "nz_batch_submit: null array");
+  if (jb->injector && (jb->parcels < 4 || jb->parcels > (1 << 22) || jb->parcels % 4 != 0))
+    return nz_fail(-1, "nz_batch_submit: parcel count must be a multiple of 4 in [4, 2^22]");
+  auto j = std::make_unique<Job>();
+  const size_t N = (size_t)jb->Nx * jb->Nr, NA = (size_t)(jb->Nx + 1) * (jb->Nr + 1) * 4;
+  j->Nx = jb->Nx;
+  j->Nr = jb->Nr;
+  j->mask.assign(jb->mask, jb->mask + N);
+  j->geoA.assign(jb->geoA, jb->geoA + NA);
+  j->geoG.assign(jb->geoG, jb->geoG + N * 4);
+  j->geoM.assign(jb->geoM, jb->geoM + N * 4);
+  if (jb->jWall) j->jWall.assign(jb->jWall, jb->jWall + (size_t)jb->Nx * 4);
+  j->A.assign(jb->A, jb->A + N * 4);
+  j->B.assign(jb->B, jb->B + N * 4);
+  if (jb->injector) j->tab.assign(jb->injector, jb->injector + 256 * 3 * 4);
+  j->P = jb->P;
+  j->parcels = jb->injector ? jb->parcels : 0;
+  j->probes = jb->probes;
+  j->tEnd = jb->t_end;
+  j->resTol = jb->res_tol;
+  j->minSteps = jb->min_steps;
+  j->maxSteps = jb->max_steps;
+  j->keep = jb->keep_state != 0;
+  j->bytes = nz::inst_bytes(jb->Nx, jb->Nr, b->prec, j->parcels > 0 ? j->parcels : 4);
+  if (j->bytes > b->budget) return nz_fail(-7, "nz_batch_submit: the job alone exceeds the memory budget");
+  if (jb->probes.dx > 0) {
+    const nz_probes& pr = jb->probes;
+    if (pr.iExit < 0 || pr.iExit + 1 > jb->Nx || pr.jExit < 0 || pr.jExit > jb->Nr || pr.iThroat < 0 ||
+        pr.iThroat >= jb->Nx || pr.iChamber < 0 || pr.iChamber >= jb->Nx)
+      return nz_fail(-1, "nz_batch_submit: probe outside the grid");
+  }
+  j->tSubmit = now_s();
+  j->id = b->nextId++;
+  *id = j->id;
+  b->queue.push_back(std::move(j));
+  return 0;
+}
+
+int nz_batch_run(nz_batch* b, int steps, int* resident, int* queued) {
+  if (!b) return nz_fail(-1, "nz_batch_run: null batch");
+  if (steps < 0) return nz_fail(-1, "nz_batch_run: negative step count");
+  b->finished.clear();
+  if (int rc = admit_queued(b)) return rc;
+  if (b->tableDirty) { if (int rc = upload_table(b)) return rc; }
+  const int n = (int)b->res.size();
+  if (n > 0 && steps > 0) {
+    if (int rc = run_steps(b, steps)) return rc;
+    HIPCHK(hipMemcpyAsync(b->stHost, b->st, sizeof(InstState) * b->maxRes, hipMemcpyDeviceToHost, b->stream), "batch poll");
+    HIPCHK(hipStreamSynchronize(b->stream), "batch run");
+    for (int k = n - 1; k >= 0; --k) {
+      const InstState& is = b->stHost[b->res[k]->slot];
+      if (is.flags & 1u) { if (int rc = retire(b, (size_t)k, is)) return rc; }
+    }
+  }
+  if (resident) *resident = (int)b->res.size();
+  if (queued) *queued = (int)b->queue.size();
+  return 0;
+}
+
+int nz_batch_result_get(nz_batch* b, int64_t id, nz_batch_result* out) {
+  if (!b || !out) return nz_fail(-1, "nz_batch_result_get: null argument");
+  std::memset(out, 0, sizeof(*out));
+  auto it = b->done.find(id);
+  if (it != b->done.end()) { *out = it->second.r; return 0; }
+  for (auto& j : b->queue)
+    if (j->id == id) { out->state = NZ_JOB_QUEUED; out->t_submit = j->tSubmit; return 0; }
+  if (Resident* r = find_res(b, id)) {
+    InstState is{};
+    HIPCHK(hipStreamSynchronize(b->stream), "batch result sync");
+    HIPCHK(hipMemcpy(&is, b->st + r->slot, sizeof(is), hipMemcpyDeviceToHost), "batch result");
+    out->state = NZ_JOB_RUNNING;
+    out->steps = is.steps;
+    out->t_grid = is.last[2];
+    out->dt = is.last[0];
+    out->residual = is.resLast;
+    out->t_submit = r->job->tSubmit;
+    out->t_admit = r->tAdmit;
+    return 0;
+  }
+  return nz_fail(-1, "nz_batch_result_get: unknown job id");
+}
+
+int nz_batch_read_state(nz_batch* b, int64_t id, float* A, float* B) {
+  if (!b || !A || !B) return nz_fail(-1, "nz_batch_read_state: null argument");
+  auto it = b->done.find(id);
+  if (it != b->done.end()) {
+    if (it->second.A.empty()) return nz_fail(-4, "nz_batch_read_state: the job was not submitted with keep_state");
+    std::memcpy(A, it->second.A.data(), it->second.A.size() * 4);
+    std::memcpy(B, it->second.B.data(), it->second.B.size() * 4);
+    return 0;
+  }
+  Resident* r = find_res(b, id);
+  if (!r) return nz_fail(-1, "nz_batch_read_state: job is not resident or finished");
+  const long long N = (long long)r->I.Nx * r->I.Nr;
+  float* tmp = nullptr;
+  HIPCHK(hipMalloc(&tmp, (size_t)N * 32), "batch read staging");
+  const bool fp64 = b->prec == 64;
+  hipError_t e = nzk::launch_pack(fp64, r->I.U0, r->I.Ns, r->I.Nx, 0, 0, 0, r->I.Nx, r->I.Nr, tmp, b->stream);
+  if (e == hipSuccess) e = nzk::launch_pack(fp64, r->I.U0, r->I.Ns, r->I.Nx, 4, 0, 0, r->I.Nx, r->I.Nr, tmp + N * 4, b->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(A, tmp, (size_t)N * 16, hipMemcpyDeviceToHost, b->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(B, tmp + N * 4, (size_t)N * 16, hipMemcpyDeviceToHost, b->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+  (void)hipFree(tmp);
+  if (e != hipSuccess) return nz_fail(-10, std::string("nz_batch_read_state: ") + hipGetErrorString(e));
+  return 0;
+}
+
+int nz_batch_release(nz_batch* b, int64_t id) {
+  if (!b) return nz_fail(-1, "nz_batch_release: null batch");
+  if (!b->done.erase(id)) return nz_fail(-1, "nz_batch_release: job is not finished");
+  return 0;
+}
+
+int nz_batch_set_graph(nz_batch* b, int on) {
+  if (!b) return nz_fail(-1, "nz_batch_set_graph: null batch");
+  b->useGraph = on != 0;
+  drop_graph(b);
+  return 0;
+}
+
+int nz_batch_memory(const nz_batch* b, uint64_t* used, uint64_t* budget) {
+  if (!b) return nz_fail(-1, "nz_batch_memory: null batch");
+  if (used) *used = b->used;
+  if (budget) *budget = b->budget;
+  return 0;
+}
+
+int nz_batch_finished(const nz_batch* b, int64_t* ids, int cap) {
+  if (!b) return nz_fail(-1, "nz_batch_finished: null batch");
+  const int n = (int)b->finished.size();
+  for (int k = 0; k < n && k < cap && ids; ++k) ids[k] = b->finished[k];
+  return n;
+}
+
+}  // extern "C"

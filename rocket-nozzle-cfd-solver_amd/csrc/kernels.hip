@@ -19,6 +19,8 @@
 // every resident solve (batched serving uses the same kernels).
 #include "nz_device.h"
 
+#include <algorithm>
+
 namespace nzk {
 
 // ------------------------------------------------------------------------------
@@ -173,11 +175,13 @@ struct StageSmem {
       R Vr[7][TY + 1][TX];
     } vis;
   } u;
-  static constexpr int MAXE = 192;                // extra faces of merged cells outside the tile
-  static constexpr int MAXF = 64;                 // cells those faces reach beyond the halo
+  static constexpr int MAXE = MAX_EXTRA_FACES;     // faces of merged cells outside the tile
+  static constexpr int MAXF = MAX_FAR_CELLS;       // cells those faces reach beyond the halo
+  static constexpr int WW = TX + 2 * FAR_HALO, WH = TY + 2 * FAR_HALO;
   uint2 ex[MAXE];
   int2 fc[MAXF];
   R PF[8][MAXF];
+  unsigned char fmap[WH][WW];                      // far-cell slot by window position
   int nEx, nF;
   double red[16];
   double redd[16];
@@ -185,7 +189,7 @@ struct StageSmem {
 };
 
 template <typename R, int TX, int TY, int STAGE, bool PHYS>
-__global__ void __launch_bounds__(TX* TY) k_stage(const DevInst* __restrict__ insts, int ninst, unsigned* __restrict__ done) {
+__global__ void __launch_bounds__(TX* TY) k_stage(const DevInst* __restrict__ insts, int ninst) {
   constexpr int NT = TX * TY;
   using S = StageSmem<R, TX, TY>;
   __shared__ S sm;
@@ -239,6 +243,15 @@ __global__ void __launch_bounds__(TX* TY) k_stage(const DevInst* __restrict__ in
       sm.PF[0][e] = q.r; sm.PF[1][e] = q.u; sm.PF[2][e] = q.v; sm.PF[3][e] = q.p;
       sm.PF[4][e] = q.Z; sm.PF[5][e] = q.F; sm.PF[6][e] = q.G; sm.PF[7][e] = q.O;
     }
+    if (nF > 0) {  // uniform over the workgroup
+      for (int w = tid; w < S::WW * S::WH; w += NT) sm.fmap[w / S::WW][w % S::WW] = 0xFF;
+      __syncthreads();
+      for (int e = tid; e < nF; e += NT) {
+        const int2 fcell = I.fcCells[fcBase + e];
+        const int wi = fcell.x - (i0 - FAR_HALO), wj = fcell.y - (j0 - FAR_HALO);
+        if (wi >= 0 && wi < S::WW && wj >= 0 && wj < S::WH) sm.fmap[wj][wi] = (unsigned char)e;
+      }
+    }
     R* __restrict__ exFlux = static_cast<R*>(I.exFlux) + (long long)exBase * 16;
     auto P_at = [&](int li, int lj) {
       Prim<R> q;
@@ -248,13 +261,15 @@ __global__ void __launch_bounds__(TX* TY) k_stage(const DevInst* __restrict__ in
     };
     // primitives by global cell index: the LDS tile, which always covers in-tile faces
     auto tileP = [&](int gi, int gj) { return P_at(gi - (i0 - 2), gj - (j0 - 2)); };
-    // ... and with a global fall-back, for faces of merged cells outside the tile
+    // ... and, for faces of merged cells outside the tile, the preloaded far cells
+    // (slot from the window map), else global memory
     auto anyP = [&](int gi, int gj) {
       const int li = gi - (i0 - 2), lj = gj - (j0 - 2);
       if (li >= 0 && li < S::PW && lj >= 0 && lj < S::PH) return P_at(li, lj);
-      for (int e = 0; e < nF; ++e) {
-        const int2 fcell = sm.fc[e];
-        if (fcell.x == gi && fcell.y == gj) {
+      const int wi = gi - (i0 - FAR_HALO), wj = gj - (j0 - FAR_HALO);
+      if (nF > 0 && wi >= 0 && wi < S::WW && wj >= 0 && wj < S::WH) {
+        const int e = sm.fmap[wj][wi];
+        if (e < nF) {
           Prim<R> q;
           q.r = sm.PF[0][e]; q.u = sm.PF[1][e]; q.v = sm.PF[2][e]; q.p = sm.PF[3][e];
           q.Z = sm.PF[4][e]; q.F = sm.PF[5][e]; q.G = sm.PF[6][e]; q.O = sm.PF[7][e];
@@ -291,25 +306,23 @@ __global__ void __launch_bounds__(TX* TY) k_stage(const DevInst* __restrict__ in
     }
     __syncthreads();
 
-    // a face's weighted inviscid flux: in-tile array, extra-face array, or computed here
-    auto invFace = [&](bool dirR, int fi, int fj, R* F9) {
+    // weighted inviscid flux of face f (W, E, S, N = 0..3) of member cell (mi, mj):
+    // the in-tile array for faces of the tile, the slave's extra-face slots (`slot`,
+    // advanced per face taken) for the others, or evaluated here when it has none
+    auto invFace = [&](int f, int mi_, int mj_, int& slot, R* F9) {
+      const bool dirR = f >= 2;
+      const int fi = mi_ + (f == 1 ? 1 : 0), fj = mj_ + (f == 3 ? 1 : 0);
       const int fx = fi - i0, fy = fj - j0;
       if (!dirR && fx >= 0 && fx <= TX && fy >= 0 && fy < TY) {
         for (int q = 0; q < 9; ++q) F9[q] = sm.u.inv.Fx[q][fy][fx];
-        return;
-      }
-      if (dirR && fx >= 0 && fx < TX && fy >= 0 && fy <= TY) {
+      } else if (dirR && fx >= 0 && fx < TX && fy >= 0 && fy <= TY) {
         for (int q = 0; q < 9; ++q) F9[q] = sm.u.inv.Fr[q][fy][fx];
-        return;
+      } else if (slot >= 0 && slot < nEx) {
+        for (int q = 0; q < 9; ++q) F9[q] = exFlux[slot * 16 + q];
+        ++slot;
+      } else {
+        face_flux_w<R>(I, dirR, fi, fj, anyP, F9, F9 + 4, F9[8]);
       }
-      for (int e = 0; e < nEx; ++e) {
-        const uint2 ef = sm.ex[e];
-        if (ex_dir(ef) == dirR && ex_i(ef) == fi && ex_j(ef) == fj) {
-          for (int q = 0; q < 9; ++q) F9[q] = exFlux[e * 16 + q];
-          return;
-        }
-      }
-      face_flux_w<R>(I, dirR, fi, fj, anyP, F9, F9 + 4, F9[8]);
     };
 
     // ---- 3. merged-group finite-volume update (FS_UPDATE)
@@ -323,17 +336,25 @@ __global__ void __launch_bounds__(TX* TY) k_stage(const DevInst* __restrict__ in
     R A[4], B[4];         // group state after the stage
     R ubarM[5];           // member-specific B.w (inflow-speed average in column 0)
     int mi[5], mj[5], nm = 0;
+    int mslot[5];         // first extra-face slot of each member (-1: faces in the tile or none)
     bool guard = false;   // NaN guard hit
     R rho_old = R(0);
     float4 gg = make_float4(0.f, 0.f, 0.f, 0.f);
     Prim<R> qS;           // state after the stage, as primitives (physics input)
     if (host) {
-      mi[0] = ci; mj[0] = cj; nm = 1;
+      mi[0] = ci; mj[0] = cj; mslot[0] = -1; nm = 1;
       const uint32_t sl = code_slaves(code);
       if (sl & 1u) { mi[nm] = ci - 1; mj[nm] = cj; ++nm; }
       if (sl & 2u) { mi[nm] = ci + 1; mj[nm] = cj; ++nm; }
       if (sl & 4u) { mi[nm] = ci; mj[nm] = cj - 1; ++nm; }
       if (sl & 8u) { mi[nm] = ci; mj[nm] = cj + 1; ++nm; }
+      // slaves outside the tile take 3 consecutive extra-face slots each, in member order
+      int es = code_exslot(code) * 3;
+      for (int m = 1; m < nm; ++m) {
+        const bool inT = mi[m] >= i0 && mi[m] < i0 + TX && mj[m] >= j0 && mj[m] < j0 + TY;
+        mslot[m] = (inT || es < 0) ? -1 : es;
+        if (!inT && es >= 0) es += 3;
+      }
       R L[4], LY[4], LO;
       for (int q = 0; q < 4; ++q) {
         L[q] = -(sm.u.inv.Fx[q][ty][tx + 1] - sm.u.inv.Fx[q][ty][tx]) - (sm.u.inv.Fr[q][ty + 1][tx] - sm.u.inv.Fr[q][ty][tx]);
@@ -343,10 +364,11 @@ __global__ void __launch_bounds__(TX* TY) k_stage(const DevInst* __restrict__ in
       for (int m = 1; m < nm; ++m) {
         // the slave's own net flux: west/south faces enter, east/north faces leave
         R Fw[9], Fe_[9], Fs[9], Fn[9];
-        invFace(false, mi[m], mj[m], Fw);
-        invFace(false, mi[m] + 1, mj[m], Fe_);
-        invFace(true, mi[m], mj[m], Fs);
-        invFace(true, mi[m], mj[m] + 1, Fn);
+        int slot = mslot[m];
+        invFace(0, mi[m], mj[m], slot, Fw);
+        invFace(1, mi[m], mj[m], slot, Fe_);
+        invFace(2, mi[m], mj[m], slot, Fs);
+        invFace(3, mi[m], mj[m], slot, Fn);
         for (int q = 0; q < 4; ++q) {
           L[q] += -(Fe_[q] - Fw[q]) - (Fn[q] - Fs[q]);
           LY[q] += -(Fe_[4 + q] - Fw[4 + q]) - (Fn[4 + q] - Fs[4 + q]);
@@ -414,7 +436,7 @@ __global__ void __launch_bounds__(TX* TY) k_stage(const DevInst* __restrict__ in
     }
 
     // ---- 4. viscous / wall / afterburning split step (FS_VCELL, FS_VFLUX, FS_SOURCE)
-    if (STAGE == 2 && PHYS) {
+    if (STAGE == 2 && PHYS && (c.visc || c.chem)) {  // per instance: a batch mixes both kinds
       __syncthreads();  // inviscid fluxes consumed; reuse their LDS
       for (int idx = tid; idx < S::TW * S::THh; idx += NT) {
         const int ri = idx % S::TW, rj = idx / S::TW;
@@ -467,24 +489,20 @@ __global__ void __launch_bounds__(TX* TY) k_stage(const DevInst* __restrict__ in
         }
         __syncthreads();
       }
-      auto visFace = [&](bool dirR, int fi, int fj, R* F7) {
+      auto visFace = [&](int f, int mi_, int mj_, int& slot, R* F7) {
+        const bool dirR = f >= 2;
+        const int fi = mi_ + (f == 1 ? 1 : 0), fj = mj_ + (f == 3 ? 1 : 0);
         const int fx = fi - i0, fy = fj - j0;
         if (!dirR && fx >= 0 && fx <= TX && fy >= 0 && fy < TY) {
           for (int q = 0; q < 7; ++q) F7[q] = sm.u.vis.Vx[q][fy][fx];
-          return;
-        }
-        if (dirR && fx >= 0 && fx < TX && fy >= 0 && fy <= TY) {
+        } else if (dirR && fx >= 0 && fx < TX && fy >= 0 && fy <= TY) {
           for (int q = 0; q < 7; ++q) F7[q] = sm.u.vis.Vr[q][fy][fx];
-          return;
+        } else if (slot >= 0 && slot < nEx) {
+          for (int q = 0; q < 7; ++q) F7[q] = exFlux[slot * 16 + 9 + q];
+          ++slot;
+        } else {
+          vface_w<R>(I, dirR, fi, fj, anyP, anyT, F7, F7 + 4);
         }
-        for (int e = 0; e < nEx; ++e) {
-          const uint2 ef = sm.ex[e];
-          if (ex_dir(ef) == dirR && ex_i(ef) == fi && ex_j(ef) == fj) {
-            for (int q = 0; q < 7; ++q) F7[q] = exFlux[e * 16 + 9 + q];
-            return;
-          }
-        }
-        vface_w<R>(I, dirR, fi, fj, anyP, anyT, F7, F7 + 4);
       };
       if (host) {
         const R rho = qS.r, p = qS.p, G = qS.G, velx = qS.u, vely = qS.v;
@@ -496,10 +514,11 @@ __global__ void __launch_bounds__(TX* TY) k_stage(const DevInst* __restrict__ in
         if (c.visc) {
           for (int m = 0; m < nm; ++m) {
             R Fw[7], Fe_[7], Fs[7], Fn[7];
-            visFace(false, mi[m], mj[m], Fw);
-            visFace(false, mi[m] + 1, mj[m], Fe_);
-            visFace(true, mi[m], mj[m], Fs);
-            visFace(true, mi[m], mj[m] + 1, Fn);
+            int slot = mslot[m];
+            visFace(0, mi[m], mj[m], slot, Fw);
+            visFace(1, mi[m], mj[m], slot, Fe_);
+            visFace(2, mi[m], mj[m], slot, Fs);
+            visFace(3, mi[m], mj[m], slot, Fn);
             for (int q = 0; q < 4; ++q) L[q] += -(Fe_[q] - Fw[q]) - (Fn[q] - Fs[q]);
             for (int q = 0; q < 3; ++q) LY[q] += -(Fe_[4 + q] - Fw[4 + q]) - (Fn[4 + q] - Fs[4 + q]);
           }
@@ -653,7 +672,6 @@ __global__ void __launch_bounds__(TX* TY) k_stage(const DevInst* __restrict__ in
     const double rb = block_sum<NT>(rloc, sm.redd);
     if (tid == 0) I.part[lt_] = make_double2(active ? wb : 0.0, active ? rb : 0.0);
   }
-  (void)done;
 }
 
 // Closes the step of every instance (one workgroup per instance): wave-speed max
@@ -686,6 +704,11 @@ __global__ void __launch_bounds__(256) k_finalize(const DevInst* __restrict__ in
       s->head = (s->head + 4u) % (uint32_t)J.c.NP;
       s->seed = s->seed + 1u;
     }
+    // batched solves stop on simulated time or on the residual
+    if (J.c.tEnd > 0.f && s->last[2] >= (double)J.c.tEnd) s->flags |= 1u | 4u;
+    if (J.c.resTol > 0.f && s->steps >= J.c.minSteps && s->resLast <= (double)J.c.resTol) s->flags |= 1u | 2u;
+    if (J.c.maxSteps > 0 && s->steps >= J.c.maxSteps) s->flags |= 1u | 8u;
+    if (J.c.maxSteps > 0 && s->steps >= J.c.maxSteps) s->flags |= 1u | 8u;
   }
 }
 
@@ -694,11 +717,14 @@ __global__ void __launch_bounds__(256) k_finalize(const DevInst* __restrict__ in
 // (VS_SPRAY_DEP, spray.js:131-142) as float atomics into the host cell.
 template <typename R>
 __global__ void __launch_bounds__(256) k_spray(const DevInst* __restrict__ insts, int ninst) {
-  // instance of this block
-  int ii = 0;
-  for (int n = 0; n < ninst; ++n)
-    if ((int)blockIdx.x >= insts[n].pblk0 && (int)blockIdx.x < insts[n].pblk0 + insts[n].npblk) ii = n;
-  const DevInst& I = insts[ii];
+  // instance of this block: the last one whose parcel blocks start at or before it
+  int lo = 0, hi = ninst - 1;
+  while (lo < hi) {
+    const int m = (lo + hi + 1) >> 1;
+    if (insts[m].pblk0 <= (int)blockIdx.x) lo = m; else hi = m - 1;
+  }
+  const DevInst& I = insts[lo];
+  if ((int)blockIdx.x >= I.pblk0 + I.npblk) return;
   const Consts& c = I.c;
   const InstState* st = I.st;
   if ((st->flags & 1u) || !c.spray) return;
@@ -821,38 +847,30 @@ __global__ void __launch_bounds__(256) k_spray(const DevInst* __restrict__ insts
 }
 
 // ------------------------------------------------------------------------------
-// wave-speed max of the current state + step-info refresh (after setState / rescale)
+// wave-speed max of the current state (after setState / rescale): per-block partials,
+// then one workgroup folds them into the instance's snapshot
 template <typename R>
-__global__ void __launch_bounds__(256) k_cfl_init(const DevInst* __restrict__ insts, int ninst, unsigned* __restrict__ done) {
+__global__ void __launch_bounds__(256) k_cfl_partial(DevInst I) {
   __shared__ double red[16];
-  __shared__ int last;
-  const int tid = threadIdx.x;
-  // grid-stride over the cells of every instance: instance by block ranges
-  int ii = 0;
-  for (int n = 0; n < ninst; ++n)
-    if ((int)blockIdx.x >= insts[n].cblk0 && (int)blockIdx.x < insts[n].cblk0 + insts[n].ncblk) ii = n;
-  const DevInst& I = insts[ii];
   const long long N = (long long)I.Nx * I.Nr;
   const R* U = static_cast<const R*>(I.U0);
   double w = 0.0;
-  for (long long k = (long long)(blockIdx.x - I.cblk0) * 256 + tid; k < N; k += (long long)I.ncblk * 256) {
+  for (long long k = (long long)blockIdx.x * 256 + threadIdx.x; k < N; k += (long long)gridDim.x * 256) {
     if (I.code[k] & C_SOLID) continue;
     const Prim<R> q = load_prim<R>(U, I.Ns, k);
     w = fmax(w, (double)(fabs(q.u) + fabs(q.v) + R(2) * sound_speed(q)));
   }
   const double wb = block_max<256>(w, red);
-  if (tid == 0) atomicMax(&I.st->wacc, (unsigned long long)__double_as_longlong(wb));
-  __threadfence();
-  if (tid == 0) last = (atomicAdd(done, 1u) == gridDim.x - 1) ? 1 : 0;
-  __syncthreads();
-  if (last) {
-    __threadfence();
-    for (int n = tid; n < ninst; n += 256) {
-      InstState* s = insts[n].st;
-      s->wsnap = __hip_atomic_load(&s->wacc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      s->wacc = 0ull;
-    }
-    if (tid == 0) *done = 0u;
+  if (threadIdx.x == 0) I.part[blockIdx.x] = make_double2(wb, 0.0);
+}
+__global__ void __launch_bounds__(256) k_cfl_reduce(DevInst I, int nblk) {
+  __shared__ double red[16];
+  double w = 0.0;
+  for (int k = threadIdx.x; k < nblk; k += 256) w = fmax(w, I.part[k].x);
+  const double wb = block_max<256>(w, red);
+  if (threadIdx.x == 0) {
+    I.st->wsnap = (unsigned long long)__double_as_longlong(wb);
+    I.st->wacc = 0ull;
   }
 }
 
@@ -1022,16 +1040,16 @@ hipError_t launch_set_inst(DevInst* dst, const DevInst& src, hipStream_t s) {
 }
 
 template <typename R, int TX, int TY>
-static hipError_t stage_t(int stage, bool phys, const DevInst* d, int ninst, int ntiles, unsigned* done, hipStream_t s) {
+static hipError_t stage_t(int stage, bool phys, const DevInst* d, int ninst, int ntiles, hipStream_t s) {
   const dim3 g(ntiles), b(TX * TY);
-  if (stage == 1) hipLaunchKernelGGL((k_stage<R, TX, TY, 1, false>), g, b, 0, s, d, ninst, done);
-  else if (phys) hipLaunchKernelGGL((k_stage<R, TX, TY, 2, true>), g, b, 0, s, d, ninst, done);
-  else hipLaunchKernelGGL((k_stage<R, TX, TY, 2, false>), g, b, 0, s, d, ninst, done);
+  if (stage == 1) hipLaunchKernelGGL((k_stage<R, TX, TY, 1, false>), g, b, 0, s, d, ninst);
+  else if (phys) hipLaunchKernelGGL((k_stage<R, TX, TY, 2, true>), g, b, 0, s, d, ninst);
+  else hipLaunchKernelGGL((k_stage<R, TX, TY, 2, false>), g, b, 0, s, d, ninst);
   return hipGetLastError();
 }
-hipError_t launch_stage(int stage, bool phys, bool fp64, const DevInst* d, int ninst, int ntiles, unsigned* done, hipStream_t s) {
-  return fp64 ? stage_t<double, TX64, TY64>(stage, phys, d, ninst, ntiles, done, s)
-              : stage_t<float, TX32, TY32>(stage, phys, d, ninst, ntiles, done, s);
+hipError_t launch_stage(int stage, bool phys, bool fp64, const DevInst* d, int ninst, int ntiles, hipStream_t s) {
+  return fp64 ? stage_t<double, TX64, TY64>(stage, phys, d, ninst, ntiles, s)
+              : stage_t<float, TX32, TY32>(stage, phys, d, ninst, ntiles, s);
 }
 hipError_t launch_finalize(const DevInst* d, int ninst, hipStream_t s) {
   hipLaunchKernelGGL(k_finalize, dim3(ninst), dim3(256), 0, s, d, ninst);
@@ -1042,9 +1060,13 @@ hipError_t launch_spray(bool fp64, const DevInst* d, int ninst, int nblk, hipStr
   else hipLaunchKernelGGL((k_spray<float>), dim3(nblk), dim3(256), 0, s, d, ninst);
   return hipGetLastError();
 }
-hipError_t launch_cfl_init(bool fp64, const DevInst* d, int ninst, int nblk, unsigned* done, hipStream_t s) {
-  if (fp64) hipLaunchKernelGGL((k_cfl_init<double>), dim3(nblk), dim3(256), 0, s, d, ninst, done);
-  else hipLaunchKernelGGL((k_cfl_init<float>), dim3(nblk), dim3(256), 0, s, d, ninst, done);
+hipError_t launch_cfl_init(bool fp64, const DevInst& I, hipStream_t s) {
+  // as many blocks as the instance has tiles (its partials array holds one per tile)
+  const long long N = (long long)I.Nx * I.Nr;
+  const int nblk = (int)std::min<long long>(std::min<long long>((N + 255) / 256, 1024), (long long)I.tilesX * I.tilesY);
+  if (fp64) hipLaunchKernelGGL((k_cfl_partial<double>), dim3(nblk), dim3(256), 0, s, I);
+  else hipLaunchKernelGGL((k_cfl_partial<float>), dim3(nblk), dim3(256), 0, s, I);
+  hipLaunchKernelGGL(k_cfl_reduce, dim3(1), dim3(256), 0, s, I, nblk);
   return hipGetLastError();
 }
 hipError_t launch_rescale(bool fp64, const DevInst& I, float fp, float fT, float du, hipStream_t s) {

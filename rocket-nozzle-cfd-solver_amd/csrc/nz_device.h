@@ -26,10 +26,17 @@ enum : uint32_t {
   C_IS_SLAVE = 1u << 6,
   C_HX_SHIFT = 8,      // host offset dx + 1 (2 bits)
   C_HY_SHIFT = 10,     // host offset dy + 1 (2 bits)
+  C_EXS_SHIFT = 12,    // bits 12..19 of a host: 1 + slot of its first slave outside the tile (0: none listed)
 };
+// per tile of the stage kernels: faces of merged cells outside the tile (3 per such
+// slave, listed host by host) and the cells their stencils reach beyond the 2-cell halo
+constexpr int MAX_EXTRA_FACES = 192;
+constexpr int MAX_FAR_CELLS = 64;
+constexpr int FAR_HALO = 4;   // far cells are looked up in a window this many cells around the tile
 __device__ __forceinline__ int code_hdx(uint32_t c) { return (int)((c >> C_HX_SHIFT) & 3u) - 1; }
 __device__ __forceinline__ int code_hdy(uint32_t c) { return (int)((c >> C_HY_SHIFT) & 3u) - 1; }
 __device__ __forceinline__ uint32_t code_slaves(uint32_t c) { return (c >> C_SLAVE_SHIFT) & 15u; }
+__device__ __forceinline__ int code_exslot(uint32_t c) { return (int)((c >> C_EXS_SHIFT) & 255u) - 1; }
 
 // ---- parameters of one instance (the reference's P block plus derived uniforms)
 struct Consts {
@@ -48,6 +55,10 @@ struct Consts {
   float resScale;  // residual normalisation 1/rho_chamber
   int spray;       // spray stepped this step
   int NP;          // parcels
+  // batched solves: an instance stops (flags bit 0) at t >= tEnd (grid time), after
+  // minSteps steps at residual <= resTol, or after maxSteps steps; zero disables a test
+  float tEnd, resTol;
+  int minSteps, maxSteps;
 };
 
 // per-instance mutable scalars kept on the device between launches
@@ -62,7 +73,7 @@ struct InstState {
   double resAcc;   // sum of squares of d(rho)/dt over groups, this step
   double resLast;  // RMS of the last completed step
   double nGroups;  // number of fluid groups (residual normalisation)
-  uint32_t flags;  // bit 0: instance inactive (skipped)
+  uint32_t flags;  // bit 0: instance inactive (skipped); stopped on bit 1: the residual, 2: time, 3: step count
   uint32_t pad;
 };
 

@@ -12,10 +12,10 @@ constexpr int TX64 = 32, TY64 = 4;   // FP64: half-width tiles keep LDS under 64
 
 void tile_shape(bool fp64, int* tx, int* ty);
 hipError_t launch_set_inst(DevInst* dst, const DevInst& src, hipStream_t s);
-hipError_t launch_stage(int stage, bool phys, bool fp64, const DevInst* d, int ninst, int ntiles, unsigned* done, hipStream_t s);
+hipError_t launch_stage(int stage, bool phys, bool fp64, const DevInst* d, int ninst, int ntiles, hipStream_t s);
 hipError_t launch_finalize(const DevInst* d, int ninst, hipStream_t s);
 hipError_t launch_spray(bool fp64, const DevInst* d, int ninst, int nblk, hipStream_t s);
-hipError_t launch_cfl_init(bool fp64, const DevInst* d, int ninst, int nblk, unsigned* done, hipStream_t s);
+hipError_t launch_cfl_init(bool fp64, const DevInst& I, hipStream_t s);
 hipError_t launch_rescale(bool fp64, const DevInst& I, float fp, float fT, float du, hipStream_t s);
 hipError_t launch_wall_row(bool fp64, const DevInst& I, const float* jw, float* out, hipStream_t s);
 hipError_t launch_t2(bool fp64, const DevInst& I, float* out, hipStream_t s);

@@ -93,6 +93,24 @@ class NzIntegrals(C.Structure):
                 ("F", "mdot", "Me", "pe", "Isp", "mdotThroat", "Pc", "Tc", "mass", "energy", "residual")]
 
 
+class NzBatchJob(C.Structure):
+    _fields_ = [("Nx", C.c_int32), ("Nr", C.c_int32), ("mask", C.POINTER(C.c_uint8)),
+                ("geoA", C.POINTER(C.c_float)), ("geoG", C.POINTER(C.c_float)), ("geoM", C.POINTER(C.c_float)),
+                ("jWall", C.POINTER(C.c_float)), ("A", C.POINTER(C.c_float)), ("B", C.POINTER(C.c_float)),
+                ("P", NzParams), ("injector", C.POINTER(C.c_float)), ("parcels", C.c_int32),
+                ("probes", NzProbes), ("t_end", C.c_double), ("res_tol", C.c_double), ("min_steps", C.c_int32),
+                ("max_steps", C.c_int32), ("keep_state", C.c_int32)]
+
+
+class NzBatchResult(C.Structure):
+    _fields_ = [("state", C.c_int32), ("stop", C.c_int32), ("steps", C.c_int64), ("t_grid", C.c_double),
+                ("dt", C.c_double), ("residual", C.c_double), ("t_submit", C.c_double), ("t_admit", C.c_double),
+                ("t_done", C.c_double), ("have_integrals", C.c_int32), ("integ", NzIntegrals)]
+
+
+NZ_JOB_QUEUED, NZ_JOB_RUNNING, NZ_JOB_DONE = range(3)
+NZ_STOP_NONE, NZ_STOP_RESIDUAL, NZ_STOP_TIME, NZ_STOP_STEPS = range(4)
+
 NZ_FIELD_A, NZ_FIELD_B, NZ_FIELD_STEP, NZ_FIELD_WALLROW, NZ_FIELD_T2 = range(5)
 NZ_BELL, NZ_CONE = 0, 1
 NZ_START_RAMP, NZ_START_BURST = 0, 1
@@ -138,6 +156,17 @@ SIGNATURES = {
     "nz_spray_read": (C.c_int, [vp, fp, fp]),
     "nz_set_probes": (C.c_int, [vp, P(NzProbes)]),
     "nz_integrals_read": (C.c_int, [vp, P(NzParams), P(NzIntegrals)]),
+    "nz_batch_create": (C.c_int, [P(vp), C.c_int, C.c_uint64, C.c_int]),
+    "nz_batch_destroy": (C.c_int, [vp]),
+    "nz_batch_instance_bytes": (C.c_uint64, [C.c_int, C.c_int, C.c_int, C.c_int]),
+    "nz_batch_submit": (C.c_int, [vp, P(NzBatchJob), P(C.c_int64)]),
+    "nz_batch_run": (C.c_int, [vp, C.c_int, P(C.c_int), P(C.c_int)]),
+    "nz_batch_result_get": (C.c_int, [vp, C.c_int64, P(NzBatchResult)]),
+    "nz_batch_read_state": (C.c_int, [vp, C.c_int64, fp, fp]),
+    "nz_batch_release": (C.c_int, [vp, C.c_int64]),
+    "nz_batch_set_graph": (C.c_int, [vp, C.c_int]),
+    "nz_batch_memory": (C.c_int, [vp, P(C.c_uint64), P(C.c_uint64)]),
+    "nz_batch_finished": (C.c_int, [vp, P(C.c_int64), C.c_int]),
     "nz_config_default": (None, [P(NzConfig)]),
     "nz_config_preset": (C.c_int, [P(NzConfig), C.c_char_p]),
     "nz_setup_build": (C.c_int, [P(NzConfig), P(vp)]),

@@ -15,7 +15,7 @@ INCLUDE = os.path.join(REPO, "include")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
-HIP_SOURCES = ["kernels.hip", "runtime.hip"]
+HIP_SOURCES = ["kernels.hip", "instance.hip", "runtime.hip", "batch.hip"]
 CPP_SOURCES = ["setup.cpp", "errors.cpp"]
 
 # device code: no packed-FP32 instructions (they are avoided on this platform)

@@ -162,10 +162,34 @@ def test_inviscid_fp64_logic():
 
 
 def test_hot_bell_full_physics():
+    """The ambient region above the chamber at the left boundary carries a mode that
+    grows from rounding level (about x40 per 16 steps in the FP64 oracle): in FP32 the
+    oracle sits on the exact ambient balance there and the mode never starts, while any
+    other rounding sequence seeds it. v's bar is therefore 1e-4 or twice the oracle's own
+    FP32-vs-FP64 difference at this horizon, and the GPU result must also be no further
+    from the FP64 oracle than that."""
     s = SimSetup(configs.hot_bell(Nx=192))
     assert s.P.visc and s.P.chem
     g, o = run_pair(s, 120, chunks=2)
-    compare_state(g, o, 1e-4, "hot bell 192 full physics")
+    noise = fp32_noise(s, 120, "hot bell 192 full physics")
+    G = g.readAll()
+    Ao, Bo = o.state()
+    pg, po = prims(G["A"], G["B"]), prims(Ao, Bo)
+    errs = {k: rel_l2(pg[k], po[k]) for k in pg}
+    say("[parity] hot bell 192 full physics: rel-L2 " + " ".join(f"{k}={v:.2e}" for k, v in errs.items()))
+    o64 = orc.Oracle(s.Nx, s.Nr, 64)
+    o64.set_geometry(s.mask, s.geoA, s.geoG, s.geoM, s.jWall)
+    o64.set_state(*s.initial_state())
+    o64.step(120, s.P)
+    A64, B64 = o64.state_f64()
+    p64 = prims(A64, B64)
+    e64 = {k: rel_l2(pg[k], p64[k]) for k in pg}
+    say("[parity] hot bell 192 full physics: GPU FP32 vs oracle FP64 " + " ".join(f"{k}={v:.2e}" for k, v in e64.items()))
+    for k in ("rho", "u", "p"):
+        assert errs[k] <= 1e-4, f"{k} rel-L2 {errs[k]:.3e}"
+    bar = max(1e-4, 2 * noise["v"])
+    assert errs["v"] <= bar and e64["v"] <= bar, (errs["v"], e64["v"], bar)
+    np.testing.assert_allclose(g.readStepInfo(), o.step_info(), rtol=1e-5)
 
 
 def test_hot_bell_fp64_logic():
