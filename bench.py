@@ -66,6 +66,158 @@ def cpu_baseline(setup, budget_s=15.0):
             "sample": f"oracle FP32 on {setup.Nx}x{setup.Nr}, {steps} steps in {dt:.1f} s ({cpu_model()})"}
 
 
+def pct(v, q):
+    import numpy as np
+    return float(np.percentile(np.asarray(v, float), q)) if len(v) else None
+
+
+def plain_one_at_a_time(items):
+    """The same solves through GPUSolver, one after another, each stepped exactly the
+    step count its batched run took (nz_step in chunks, one synchronisation at the end),
+    so the plain path does no stop-test polling at all."""
+    from rnozzle.solver import GPUSolver, load_setup
+    total, cu = 0.0, 0.0
+    for s, steps in items:
+        g = GPUSolver()
+        if s.injector is not None:
+            g.resize(s.Nx, s.Nr)
+            g.spray.setCount(16384)
+        load_setup(g, s)
+        g.sync()
+        t0 = time.perf_counter()
+        left = steps
+        while left > 0:
+            n = min(left, 1000)
+            g.step(n, s.P)
+            left -= n
+        g.sync()
+        total += time.perf_counter() - t0
+        cu += s.Nx * s.Nr * steps
+        g.close()
+    return total, cu
+
+
+def bench_batch(args):
+    """Batched solve set: `--instances` nozzles drawn from `--seed`, each to a residual
+    of 1e-6 or 3 ms simulated, all resident and advancing in the same launches;
+    against the same solves one at a time through GPUSolver."""
+    import numpy as np
+    from rnozzle import configs
+    from rnozzle.batch import Batch
+    from rnozzle.sim_setup import SimSetup
+    rng = np.random.default_rng(args.seed)
+    t0 = time.perf_counter()
+    setups = [SimSetup(configs.batch_instance(rng)) for _ in range(args.instances)]
+    t_setup = time.perf_counter() - t0
+    b = Batch(max_resident=max(1, args.instances), budget_bytes=int(args.budget_gb * (1 << 30)))
+    ids = [b.submit(s, t_end_s=configs.BATCH_T_END, res_tol=configs.BATCH_RES_TOL, min_steps=100) for s in setups]
+    t0 = time.perf_counter()
+    runs = 0
+    while True:
+        r, q = b.run(args.chunk)
+        runs += 1
+        if runs % 20 == 0:
+            print(f"[batch] {time.perf_counter() - t0:.1f} s: {r} resident, {q} queued", file=sys.stderr, flush=True)
+        if r == 0 and q == 0:
+            break
+    wall = time.perf_counter() - t0
+    res = [b.result(j) for j in ids]
+    steps = [r["steps"] for r in res]
+    cu = sum(s.Nx * s.Nr * n for s, n in zip(setups, steps))
+    lat = [r["t_done"] - r["t_submit"] for r in res]
+    stops = {"residual": sum(r["stop"] == 1 for r in res), "time": sum(r["stop"] == 2 for r in res)}
+    out = {
+        "metric": "batched solves/s", "value": len(ids) / wall, "unit": "solves/s", "n_gpus": 1,
+        "steps": int(sum(steps)), "warmup": 0, "ms_per_step": None, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "fp32", "data": f"synthetic: {len(ids)} nozzles drawn from seed {args.seed}",
+        "config": {"workload": f"batch{len(ids)}", "instances": len(ids), "seed": args.seed,
+                   "t_end_s": configs.BATCH_T_END, "res_tol": configs.BATCH_RES_TOL, "chunk_steps": args.chunk,
+                   "grids": sorted({f"{s.Nx}x{s.Nr}" for s in setups})},
+        "wall_s": wall, "cell_updates_per_s": cu / wall, "setup_build_s": t_setup, "stops": stops,
+        "latency_s": {"p50": pct(lat, 50), "p99": pct(lat, 99), "max": max(lat)},
+    }
+    if args.plain != 0:
+        n = len(ids) if args.plain < 0 else min(args.plain, len(ids))
+        tp, cup = plain_one_at_a_time(list(zip(setups[:n], steps[:n])))
+        out["plain"] = {"solves": n, "wall_s": tp, "solves_per_s": n / tp, "cell_updates_per_s": cup / tp,
+                        "how": "GPUSolver per solve, same step counts, sequential"}
+        out["speedup_vs_plain"] = out["value"] / (n / tp)
+    print(json.dumps(out), flush=True)
+
+
+def bench_serve(args):
+    """Serving: Poisson arrivals (rate `--rate`/s) of `--requests` requests, 70 %
+    nozzles as in the batched set, 30 % 384x128 spray injectors with 16,384 parcels;
+    admitted by memory budget as others finish. Latency = completion - arrival."""
+    import numpy as np
+    from rnozzle import configs
+    from rnozzle.batch import Batch
+    from rnozzle.sim_setup import SimSetup
+    rng = np.random.default_rng(args.seed)
+    arrivals = np.cumsum(rng.exponential(1.0 / args.rate, args.requests))
+    reqs = []
+    for k in range(args.requests):
+        inj = rng.uniform() < 0.3
+        cfg = configs.injector_instance(rng) if inj else configs.batch_instance(rng)
+        reqs.append((float(arrivals[k]), SimSetup(cfg), inj))
+    b = Batch(max_resident=args.requests, budget_bytes=int(args.budget_gb * (1 << 30)))
+    t0 = time.perf_counter()
+    m0 = time.monotonic()
+    arrival_of = {}
+    nxt = 0
+    done = []
+    while True:
+        now = time.perf_counter() - t0
+        while nxt < len(reqs) and reqs[nxt][0] <= now:
+            a, s, inj = reqs[nxt]
+            jid = b.submit(s, t_end_s=configs.BATCH_T_END, res_tol=configs.BATCH_RES_TOL, min_steps=100,
+                           parcels=configs.INJECTOR_PARCELS if inj else None)
+            arrival_of[jid] = (a, inj)
+            nxt += 1
+        r, q = b.run(args.chunk)
+        done += b.finished()
+        if nxt == len(reqs) and r == 0 and q == 0:
+            break
+        if r == 0 and q == 0 and nxt < len(reqs):
+            time.sleep(max(0.0, reqs[nxt][0] - (time.perf_counter() - t0)))
+    wall = time.perf_counter() - t0
+    # the runner's clock is the host steady clock (CLOCK_MONOTONIC), as time.monotonic()
+    lat, lat_inj, lat_noz = [], [], []
+    for jid in done:
+        res = b.result(jid)
+        a, inj = arrival_of[jid]
+        l = res["t_done"] - (m0 + a)
+        lat.append(l)
+        (lat_inj if inj else lat_noz).append(l)
+    out = {
+        "metric": "serving solves/s", "value": len(done) / wall, "unit": "solves/s", "n_gpus": 1,
+        "steps": None, "warmup": 0, "ms_per_step": None, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "fp32",
+        "data": f"synthetic: {args.requests} Poisson arrivals at {args.rate}/s from seed {args.seed}",
+        "config": {"workload": f"serve{args.requests}", "rate_per_s": args.rate, "requests": args.requests,
+                   "injector_share": 0.3, "parcels": configs.INJECTOR_PARCELS, "t_end_s": configs.BATCH_T_END,
+                   "res_tol": configs.BATCH_RES_TOL, "chunk_steps": args.chunk},
+        "wall_s": wall,
+        "time_to_converged_s": {"p50": pct(lat, 50), "p99": pct(lat, 99),
+                                "nozzle_p50": pct(lat_noz, 50), "injector_p50": pct(lat_inj, 50)},
+    }
+    print(json.dumps(out), flush=True)
+
+
+def calibrate_copy(mib):
+    """A plain device copy of known size (torch's copy kernel), run 3 times: the
+    reference point that scales the FETCH_SIZE / WRITE_SIZE counters of a profile."""
+    import torch
+    n = mib * (1 << 20) // 4
+    x = torch.ones(n, device="cuda", dtype=torch.float32)
+    y = torch.empty_like(x)
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"calibration": "device copy", "bytes_read": n * 4, "bytes_written": n * 4, "launches": 3}),
+          flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -76,11 +228,27 @@ def main():
     ap.add_argument("--set", action="append", default=[], help="config override key=value (e.g. wallModel=stair)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--instances", type=int, default=64, help="batch: solves in the set")
+    ap.add_argument("--seed", type=int, default=20240501)
+    ap.add_argument("--chunk", type=int, default=200, help="batch/serve: steps per run call")
+    ap.add_argument("--plain", type=int, default=-1, help="batch: solves re-run one at a time (-1 all, 0 none)")
+    ap.add_argument("--budget-gb", type=float, default=64.0)
+    ap.add_argument("--rate", type=float, default=200.0, help="serve: mean arrivals per second")
+    ap.add_argument("--requests", type=int, default=64)
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="run only a device copy of MIB MiB (profiler counter calibration)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        return calibrate_copy(args.calibrate_copy)
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(instances shard across GPUs by design, see the sharding notes)", file=sys.stderr)
         sys.exit(2)
+
+    if args.workload == "batch":
+        return bench_batch(args)
+    if args.workload == "serve":
+        return bench_serve(args)
 
     from rnozzle import configs
     from rnozzle._lib import check, lib

@@ -110,13 +110,26 @@ int nz_set_state(nz_solver* s, const float* A, const float* B);
 /* GPUSolver.step(n, P) (solver.js:849-891): enqueues n steps, returns
  * without waiting for the device */
 int nz_step(nz_solver* s, int n, const nz_params* P);
-/* Instrumented nz_step (no counterpart in the reference; for benchmarks): every
- * kernel launch is bracketed by HIP events on the solver's stream; after waiting
- * for the device, ms[0..3] = summed times of the stage-1 kernel, the spray
- * kernel, the fused stage-2 kernel, and first-to-last event of the n steps. */
+/* Instrumented nz_step (no counterpart in the reference; for benchmarks): the
+ * stage-1, spray and stage-2 kernels are launched with start / stop events that
+ * record the kernels' own execution (hipExtLaunchKernel); after waiting for the
+ * device, ms[0..3] = summed kernel times of stage 1, the spray, stage 2, and the
+ * span of the n steps on the solver's stream. */
 int nz_step_timed(nz_solver* s, int n, const nz_params* P, double* ms);
 /* GPUSolver.rescaleAmbient(fp, fT, du, P) (solver.js:998-1010) */
 int nz_rescale_ambient(nz_solver* s, float fp, float fT, float du, const nz_params* P);
+/* Tile shapes of the fused stage kernels (no counterpart in the reference, whose
+ * passes are full-screen draws). nz_tile_shapes fills tx[k], ty[k] for k < cap and
+ * returns how many shapes the library holds at this precision; shape 0 is the
+ * default. The shape changes speed only, never results beyond FP rounding order
+ * (none: each face and cell is evaluated by the same arithmetic in every shape). */
+int nz_tile_shapes(int precision, int* tx, int* ty, int cap);
+int nz_set_tile_shape(nz_solver* s, int shape);
+int nz_get_tile_shape(const nz_solver* s, int* shape);
+/* Autotuner: times `steps` steps of every shape from the current state, restoring
+ * the state (fields, parcels, step info) after each trial, and keeps the fastest.
+ * us_per_step (NULL or >= nz_tile_shapes() entries) receives each shape's time. */
+int nz_autotune(nz_solver* s, const nz_params* P, int steps, int* best, double* us_per_step);
 /* GPUSolver.drawWallRow() (solver.js:986-996): fills NZ_FIELD_WALLROW */
 int nz_draw_wall_row(nz_solver* s);
 /* GPUSolver._readRect(tex, x, y, w, h) (solver.js:1016-1025):
@@ -234,6 +247,8 @@ int nz_batch_release(nz_batch* b, int64_t id);
  * resident set and step count are unchanged (default 1) */
 int nz_batch_set_graph(nz_batch* b, int on);
 int nz_batch_memory(const nz_batch* b, uint64_t* used, uint64_t* budget);
+/* tile shape of every resident and later admission (nz_tile_shapes) */
+int nz_batch_set_tile_shape(nz_batch* b, int shape);
 /* ids of the jobs that finished in the last nz_batch_run; returns the count
  * (at most cap are written) */
 int nz_batch_finished(const nz_batch* b, int64_t* ids, int cap);

@@ -84,6 +84,7 @@ struct nz_batch {
   int ntiles = 0, npblk = 0;
   bool phys = false;
   bool useGraph = true;
+  int shape = 0;                 // tile shape of every resident
   hipGraphExec_t graph = nullptr;
   int graphSteps = 0;
 };
@@ -102,7 +103,7 @@ int admit(nz_batch* b, std::unique_ptr<Job> job) {
   nz::Inst& I = r->I;
   I.NP = job->tab.empty() ? 4 : job->parcels;
   // a fresh solve: the slot's state info starts zeroed with spray seed 1 (spray.js:221)
-  if (int rc = nz::inst_alloc(I, job->Nx, job->Nr, b->prec, b->st + r->slot)) { nz::inst_free(I); return rc; }
+  if (int rc = nz::inst_alloc(I, job->Nx, job->Nr, b->prec, b->st + r->slot, b->shape)) { nz::inst_free(I); return rc; }
   int rc = nz::inst_set_geometry(I, job->mask.data(), job->geoA.data(), job->geoG.data(), job->geoM.data(),
                                  job->jWall.empty() ? nullptr : job->jWall.data());
   if (!rc && !job->tab.empty()) rc = nz::inst_set_injector(I, job->tab.data());
@@ -168,9 +169,9 @@ int enqueue_steps(nz_batch* b, int steps) {
   const bool fp64 = b->prec == 64;
   const int n = (int)b->res.size();
   for (int k = 0; k < steps; ++k) {
-    HIPCHK(nzk::launch_stage(1, false, fp64, b->dtab, n, b->ntiles, b->stream), "batch stage 1");
+    HIPCHK(nzk::launch_stage(1, false, fp64, b->shape, b->dtab, n, b->ntiles, b->stream), "batch stage 1");
     if (b->npblk) HIPCHK(nzk::launch_spray(fp64, b->dtab, n, b->npblk, b->stream), "batch spray");
-    HIPCHK(nzk::launch_stage(2, b->phys, fp64, b->dtab, n, b->ntiles, b->stream), "batch stage 2");
+    HIPCHK(nzk::launch_stage(2, b->phys, fp64, b->shape, b->dtab, n, b->ntiles, b->stream), "batch stage 2");
     HIPCHK(nzk::launch_finalize(b->dtab, n, b->stream), "batch finalize");
   }
   return 0;
@@ -420,6 +421,18 @@ int nz_batch_set_graph(nz_batch* b, int on) {
   if (!b) return nz_fail(-1, "nz_batch_set_graph: null batch");
   b->useGraph = on != 0;
   drop_graph(b);
+  return 0;
+}
+
+int nz_batch_set_tile_shape(nz_batch* b, int shape) {
+  if (!b) return nz_fail(-1, "nz_batch_set_tile_shape: null batch");
+  int tx = 0, ty = 0;
+  if (!nzk::tile_shape(b->prec == 64, shape, &tx, &ty)) return nz_fail(-1, "nz_batch_set_tile_shape: unknown tile shape");
+  HIPCHK(hipStreamSynchronize(b->stream), "nz_batch_set_tile_shape: sync");
+  for (auto& r : b->res)
+    if (int rc = nz::inst_set_shape(r->I, shape)) return rc;
+  b->shape = shape;
+  b->tableDirty = true;
   return 0;
 }
 

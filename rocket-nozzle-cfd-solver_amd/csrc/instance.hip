@@ -29,8 +29,8 @@ void vfree(void*& p) {
 }  // namespace
 
 size_t inst_bytes(int Nx, int Nr, int prec, int NP) {
-  int TX, TY;
-  nzk::tile_shape(prec == 64, &TX, &TY);
+  int TX = 0, TY = 0;
+  nzk::tile_shape(prec == 64, 0, &TX, &TY);
   const size_t N = (size_t)Nx * Nr, Ns = (N + 63) / 64 * 64, R = prec == 64 ? 8 : 4;
   const size_t tiles = (size_t)((Nx + TX - 1) / TX) * ((Nr + TY - 1) / TY);
   return Ns * 8 * R * 2                                   // U0, U1
@@ -41,15 +41,16 @@ size_t inst_bytes(int Nx, int Nr, int prec, int NP) {
          + tiles * (16 + 8) + (size_t)Nx * 32 + 4096;     // partials, starts, wall row, small
 }
 
-int inst_alloc(Inst& I, int Nx, int Nr, int prec, InstState* st) {
+int inst_alloc(Inst& I, int Nx, int Nr, int prec, InstState* st, int shape) {
   inst_free(I);
+  if (!nzk::tile_shape(prec == 64, shape, &I.TX, &I.TY)) return nz_fail(-1, "unknown tile shape");
+  I.shape = shape;
   I.prec = prec;
   I.rsz = prec == 64 ? 8 : 4;
   I.Nx = Nx;
   I.Nr = Nr;
   const long long N = (long long)Nx * Nr;
   I.Ns = (N + 63) / 64 * 64;
-  nzk::tile_shape(prec == 64, &I.TX, &I.TY);
   I.tilesX = (Nx + I.TX - 1) / I.TX;
   I.tilesY = (Nr + I.TY - 1) / I.TY;
   const size_t tiles = (size_t)I.tilesX * I.tilesY, R = I.rsz;
@@ -96,6 +97,7 @@ int inst_alloc(Inst& I, int Nx, int Nr, int prec, InstState* st) {
   is.seed = 1;  // GPUSpray starts at seed 1 (spray.js:221)
   HIPCHK(hipMemcpy(I.st, &is, sizeof(is), hipMemcpyHostToDevice), "init state info");
   I.mask.assign((size_t)N, 0);
+  I.codeH.assign((size_t)N, kZeroOffset);
   I.jWall.assign((size_t)Nx * 4, 0.f);
   I.haveGeo = I.haveTab = false;
   I.bytes = inst_bytes(Nx, Nr, prec, I.NP);
@@ -276,6 +278,7 @@ int inst_set_geometry(Inst& I, const uint8_t* mask, const float* geoA, const flo
   HIPCHK(hipMemcpy(I.gg, gg.data(), gg.size() * 16, hipMemcpyHostToDevice), "upload gg");
   HIPCHK(hipMemcpy(I.code, code.data(), code.size() * 4, hipMemcpyHostToDevice), "upload code");
   HIPCHK(hipMemcpy(I.lt, lt.data(), lt.size() * 4, hipMemcpyHostToDevice), "upload lt");
+  I.codeH = std::move(code);
   I.nGroups = groups;
   HIPCHK(hipMemcpy(&I.st->nGroups, &I.nGroups, sizeof(double), hipMemcpyHostToDevice), "upload groups");
   std::memcpy(I.mask.data(), mask, (size_t)N);
@@ -289,11 +292,32 @@ int inst_set_geometry(Inst& I, const uint8_t* mask, const float* geoA, const flo
 
 int inst_set_mask(Inst& I, const uint8_t* mask) {
   const long long N = (long long)I.Nx * I.Nr;
-  std::vector<uint32_t> code((size_t)N);
-  HIPCHK(hipMemcpy(code.data(), I.code, (size_t)N * 4, hipMemcpyDeviceToHost), "read code");
+  std::vector<uint32_t>& code = I.codeH;
   for (long long k = 0; k < N; ++k) code[k] = mask[k] ? (code[k] | nzk::C_SOLID) : (code[k] & ~nzk::C_SOLID);
   HIPCHK(hipMemcpy(I.code, code.data(), (size_t)N * 4, hipMemcpyHostToDevice), "upload code");
   std::memcpy(I.mask.data(), mask, (size_t)N);
+  return 0;
+}
+
+int inst_set_shape(Inst& I, int shape) {
+  int TX = 0, TY = 0;
+  if (!nzk::tile_shape(I.prec == 64, shape, &TX, &TY)) return nz_fail(-1, "unknown tile shape");
+  if (shape == I.shape) return 0;
+  I.shape = shape;
+  I.TX = TX;
+  I.TY = TY;
+  I.tilesX = (I.Nx + TX - 1) / TX;
+  I.tilesY = (I.Nr + TY - 1) / TY;
+  const size_t tiles = (size_t)I.tilesX * I.tilesY;
+  hfree(I.part);
+  hfree(I.exStart);
+  hfree(I.fcStart);
+  HIPCHK(hipMalloc(&I.part, tiles * sizeof(double2)), "hipMalloc partials");
+  HIPCHK(hipMalloc(&I.exStart, (tiles + 1) * sizeof(int)), "hipMalloc extra starts");
+  HIPCHK(hipMalloc(&I.fcStart, (tiles + 1) * sizeof(int)), "hipMalloc far starts");
+  HIPCHK(hipMemset(I.part, 0, tiles * sizeof(double2)), "memset");
+  if (int rc = build_extra_faces(I, I.codeH)) return rc;
+  HIPCHK(hipMemcpy(I.code, I.codeH.data(), I.codeH.size() * 4, hipMemcpyHostToDevice), "upload code");
   return 0;
 }
 

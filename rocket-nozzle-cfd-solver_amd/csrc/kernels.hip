@@ -17,663 +17,14 @@
 // the wave-speed max and the previous step info (FS_STEPINFO), so dt never
 // leaves the device. Launches carry a table of instances: one launch advances
 // every resident solve (batched serving uses the same kernels).
-#include "nz_device.h"
+#include "nz_stage.h"
+#include "stage_shapes.h"
+
+#include <hip/hip_ext.h>
 
 #include <algorithm>
 
 namespace nzk {
-
-// ------------------------------------------------------------------------------
-// instance lookup for a flattened tile index (instances sorted by tile0)
-__device__ __forceinline__ int find_inst(const DevInst* __restrict__ I, int n, int t) {
-  int lo = 0, hi = n - 1;
-  while (lo < hi) {
-    const int m = (lo + hi + 1) >> 1;
-    if (I[m].tile0 <= t) lo = m; else hi = m - 1;
-  }
-  return lo;
-}
-
-// FS_STEPINFO (solver.js:404-409): dt from the wave-speed max of the current state,
-// in the arithmetic type of the state
-template <typename R>
-__device__ __forceinline__ void step_info(const Consts& c, const InstState& s, R* cur) {
-  const R w = fmax(R(c.wFloor), R(__longlong_as_double((long long)s.wsnap)));
-  R dt = fmin(R(c.cfl) / w, R(c.dtMax));
-  const R lastDt = R(s.last[0]), lastT = R(s.last[2]);
-  if (lastDt > R(0)) dt = fmin(dt, lastDt * R(1.2f));
-  cur[0] = dt;
-  cur[1] = lastT;
-  cur[2] = lastT + dt;
-  cur[3] = w;
-}
-
-template <int NT>
-__device__ __forceinline__ double block_max(double v, double* red) {
-  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  if (lane == 0) red[wv] = v;
-  __syncthreads();
-  double m = 0.0;
-  if (threadIdx.x == 0) {
-    for (int k = 0; k < NT / 64; ++k) m = fmax(m, red[k]);
-  }
-  return m;  // valid on thread 0
-}
-template <int NT>
-__device__ __forceinline__ double block_sum(double v, double* red) {
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  if (lane == 0) red[wv] = v;
-  __syncthreads();
-  double m = 0.0;
-  if (threadIdx.x == 0) {
-    for (int k = 0; k < NT / 64; ++k) m += red[k];
-  }
-  return m;
-}
-
-// ------------------------------------------------------------------------------
-// One face's inviscid flux times |area| (FS_FLUX_X/R for the face, as memberFlux
-// weights it), with primitives from `getP(i, j)`: the LDS tile, or the tile with a
-// global fall-back for faces of merged cells that lie outside it.
-template <typename R, class GetP>
-__device__ __forceinline__ void face_flux_w(const DevInst& I, bool dirR, int fi, int fj, GetP getP, R* F, R* FY, R& FO) {
-  for (int q = 0; q < 4; ++q) { F[q] = R(0); FY[q] = R(0); }
-  FO = R(0);
-  const int Nx = I.Nx, Nr = I.Nr;
-  const bool inDom = dirR ? (fi >= 0 && fi < Nx && fj >= 0 && fj <= Nr) : (fi >= 0 && fi <= Nx && fj >= 0 && fj < Nr);
-  if (!inDom) return;
-  const float area = dirR ? I.ar[(long long)fj * Nx + fi] : I.ax[(long long)fj * (Nx + 1) + fi];
-  if (area == 0.f || (dirR && fj == 0)) return;
-  const int dx = dirR ? 0 : 1, dy = dirR ? 1 : 0, tx = dy, ty = dx;
-  Prim<R> c0 = getP(fi - 2 * dx, fj - 2 * dy), c1 = getP(fi - dx, fj - dy);
-  Prim<R> c2 = getP(fi, fj), c3 = getP(fi + dx, fj + dy);
-  const bool s1 = c1.r < R(0), s2 = c2.r < R(0);
-  if (s1 && s2) return;
-  R hllw = R(0);
-  if (I.c.hllBlend > 0.f && !(s1 || s2)) {
-    const Prim<R> a1 = getP(fi - dx + tx, fj - dy + ty), b1 = getP(fi - dx - tx, fj - dy - ty);
-    const Prim<R> a2 = getP(fi + tx, fj + ty), b2 = getP(fi - tx, fj - ty);
-    hllw = hll_weight<R>(c1.p, c2.p, a1.r < R(0) ? c1.p : a1.p, b1.r < R(0) ? c1.p : b1.p,
-                         a2.r < R(0) ? c2.p : a2.p, b2.r < R(0) ? c2.p : b2.p, R(I.c.hllBlend));
-  }
-  if (dirR) {
-    R t;
-    t = c0.u; c0.u = c0.v; c0.v = t;
-    t = c1.u; c1.u = c1.v; c1.v = t;
-    t = c2.u; c2.u = c2.v; c2.v = t;
-    t = c3.u; c3.u = c3.v; c3.v = t;
-  }
-  face_flux<R>(c0, c1, c2, c3, hllw, I.c.order2 && area > 0.f, F, FY, FO);
-  if (dirR) { const R t = F[1]; F[1] = F[2]; F[2] = t; }
-  const R a = R(fabsf(area));
-  for (int q = 0; q < 4; ++q) { F[q] *= a; FY[q] *= a; }
-  FO *= a;
-}
-
-// One face's viscous flux times |area| (FS_VFLUX_X/R); getT(i, j, T1, T2) gives the
-// per-cell gradients / viscosities of an interior fluid cell.
-template <typename R, class GetP, class GetT>
-__device__ __forceinline__ void vface_w(const DevInst& I, bool dirR, int fi, int fj, GetP getP, GetT getT, R* F, R* FY) {
-  for (int q = 0; q < 4; ++q) F[q] = R(0);
-  for (int q = 0; q < 3; ++q) FY[q] = R(0);
-  const int Nx = I.Nx, Nr = I.Nr;
-  const bool on = dirR ? (fi >= 0 && fi < Nx && fj > 0 && fj < Nr) : (fi > 0 && fi < Nx && fj >= 0 && fj < Nr);
-  if (!on) return;
-  const float area = dirR ? I.ar[(long long)fj * Nx + fi] : I.ax[(long long)fj * (Nx + 1) + fi];
-  if (area == 0.f) return;
-  const int k1i = dirR ? fi : fi - 1, k1j = dirR ? fj - 1 : fj;
-  const Prim<R> w1 = getP(k1i, k1j), w2 = getP(fi, fj);
-  if (w1.r < R(0) || w2.r < R(0)) return;
-  R t1[4], m1[4], t2[4], m2[4];
-  getT(k1i, k1j, t1, m1);
-  getT(fi, fj, t2, m2);
-  visc_face<R>(dirR, fj, w1, w2, t1, t2, m1, m2, F, FY);
-  const R a = R(fabsf(area));
-  for (int q = 0; q < 4; ++q) F[q] *= a;
-  for (int q = 0; q < 3; ++q) FY[q] *= a;
-}
-
-// T1/T2 of an interior cell from primitives given by getP (zero for walls)
-template <typename R, class GetP>
-__device__ __forceinline__ void vcell_at(const DevInst& I, GetP getP, R dt, int i, int j, R* T1, R* T2) {
-  for (int q = 0; q < 4; ++q) { T1[q] = R(0); T2[q] = R(0); }
-  const Prim<R> w0 = getP(i, j);
-  if (w0.r < R(0)) return;
-  vcell<R>(w0, getP(i + 1, j), getP(i - 1, j), getP(i, j + 1), getP(i, j - 1), i, j, R(I.lt[(long long)j * I.Nx + i]), dt,
-           I.c, T1, T2);
-}
-
-// T1/T2 of an interior cell from global memory
-template <typename R>
-__device__ void vcell_global(const DevInst& I, const R* U, R t, R dt, int i, int j, R* T1, R* T2) {
-  auto g = [&](int a, int b) { return prim_at<R>(I, U, a, b, t); };
-  vcell_at<R>(I, g, dt, i, j, T1, T2);
-}
-
-// extra face of a merged cell outside the tile: (face x | dir << 31, face y)
-__device__ __forceinline__ bool ex_dir(uint2 e) { return (e.x >> 31) != 0; }
-__device__ __forceinline__ int ex_i(uint2 e) { return (int)(e.x & 0x7fffffffu); }
-__device__ __forceinline__ int ex_j(uint2 e) { return (int)e.y; }
-
-// ------------------------------------------------------------------------------
-template <typename R, int TX, int TY>
-struct StageSmem {
-  static constexpr int PW = TX + 4, PH = TY + 4;   // primitive tile with 2-cell halo
-  static constexpr int NFX = (TX + 1) * TY, NFR = TX * (TY + 1);
-  static constexpr int TW = TX + 2, THh = TY + 2;  // T1/T2 tile with 1-cell ring
-  R P[8][PH][PW];
-  union {
-    struct {
-      R Fx[9][TY][TX + 1];
-      R Fr[9][TY + 1][TX];
-    } inv;
-    struct {
-      R T[8][THh][TW];
-      R Vx[7][TY][TX + 1];
-      R Vr[7][TY + 1][TX];
-    } vis;
-  } u;
-  static constexpr int MAXE = MAX_EXTRA_FACES;     // faces of merged cells outside the tile
-  static constexpr int MAXF = MAX_FAR_CELLS;       // cells those faces reach beyond the halo
-  static constexpr int WW = TX + 2 * FAR_HALO, WH = TY + 2 * FAR_HALO;
-  uint2 ex[MAXE];
-  int2 fc[MAXF];
-  R PF[8][MAXF];
-  unsigned char fmap[WH][WW];                      // far-cell slot by window position
-  int nEx, nF;
-  double red[16];
-  double redd[16];
-  int last;
-};
-
-template <typename R, int TX, int TY, int STAGE, bool PHYS>
-__global__ void __launch_bounds__(TX* TY) k_stage(const DevInst* __restrict__ insts, int ninst) {
-  constexpr int NT = TX * TY;
-  using S = StageSmem<R, TX, TY>;
-  __shared__ S sm;
-  const int tid = threadIdx.x;
-  const int t = blockIdx.x;
-  const int ii = find_inst(insts, ninst, t);
-  const DevInst& I = insts[ii];
-  const Consts& c = I.c;
-  InstState* st = I.st;
-  const bool active = !(st->flags & 1u);
-  const int Nx = I.Nx, Nr = I.Nr;
-  const long long Ns = I.Ns;
-  const int lt_ = t - I.tile0;
-  const int i0 = (lt_ % I.tilesX) * TX, j0 = (lt_ / I.tilesX) * TY;
-  R* __restrict__ U0 = static_cast<R*>(I.U0);
-  R* __restrict__ U1 = static_cast<R*>(I.U1);
-  const R* __restrict__ Uin = STAGE == 1 ? U0 : U1;
-
-  // time step of this step: FS_STEPINFO at the start of the step
-  R cur[4];
-  if (STAGE == 1) {
-    step_info<R>(c, *st, cur);
-    if (lt_ == 0 && tid == 0 && active) { st->cur[0] = cur[0]; st->cur[1] = cur[1]; st->cur[2] = cur[2]; st->cur[3] = cur[3]; }
-  } else {
-    cur[0] = R(st->cur[0]); cur[1] = R(st->cur[1]); cur[2] = R(st->cur[2]); cur[3] = R(st->cur[3]);
-  }
-  const R dt = cur[0];
-  const R tsel = STAGE == 1 ? cur[1] : cur[2];
-
-  double wloc = 0.0;
-  double rloc = 0.0;
-  if (active) {
-    // ---- 1. primitive tile with ghost halo (FS_PRIM); this tile's extra faces
-    for (int idx = tid; idx < S::PW * S::PH; idx += NT) {
-      const int li = idx % S::PW, lj = idx / S::PW;
-      const Prim<R> q = prim_at<R>(I, Uin, i0 - 2 + li, j0 - 2 + lj, tsel);
-      sm.P[0][lj][li] = q.r; sm.P[1][lj][li] = q.u; sm.P[2][lj][li] = q.v; sm.P[3][lj][li] = q.p;
-      sm.P[4][lj][li] = q.Z; sm.P[5][lj][li] = q.F; sm.P[6][lj][li] = q.G; sm.P[7][lj][li] = q.O;
-    }
-    // extra faces of merged cells outside the tile, and the cells they reach beyond
-    // the halo (primitives loaded here in parallel, so no thread waits on them later)
-    const int exBase = I.exStart[lt_];
-    const int nEx = min(I.exStart[lt_ + 1] - exBase, S::MAXE);
-    const int fcBase = I.fcStart[lt_];
-    const int nF = min(I.fcStart[lt_ + 1] - fcBase, S::MAXF);
-    for (int e = tid; e < nEx; e += NT) sm.ex[e] = I.exFaces[exBase + e];
-    for (int e = tid; e < nF; e += NT) {
-      const int2 fcell = I.fcCells[fcBase + e];
-      sm.fc[e] = fcell;
-      const Prim<R> q = prim_at<R>(I, Uin, fcell.x, fcell.y, tsel);
-      sm.PF[0][e] = q.r; sm.PF[1][e] = q.u; sm.PF[2][e] = q.v; sm.PF[3][e] = q.p;
-      sm.PF[4][e] = q.Z; sm.PF[5][e] = q.F; sm.PF[6][e] = q.G; sm.PF[7][e] = q.O;
-    }
-    if (nF > 0) {  // uniform over the workgroup
-      for (int w = tid; w < S::WW * S::WH; w += NT) sm.fmap[w / S::WW][w % S::WW] = 0xFF;
-      __syncthreads();
-      for (int e = tid; e < nF; e += NT) {
-        const int2 fcell = I.fcCells[fcBase + e];
-        const int wi = fcell.x - (i0 - FAR_HALO), wj = fcell.y - (j0 - FAR_HALO);
-        if (wi >= 0 && wi < S::WW && wj >= 0 && wj < S::WH) sm.fmap[wj][wi] = (unsigned char)e;
-      }
-    }
-    R* __restrict__ exFlux = static_cast<R*>(I.exFlux) + (long long)exBase * 16;
-    auto P_at = [&](int li, int lj) {
-      Prim<R> q;
-      q.r = sm.P[0][lj][li]; q.u = sm.P[1][lj][li]; q.v = sm.P[2][lj][li]; q.p = sm.P[3][lj][li];
-      q.Z = sm.P[4][lj][li]; q.F = sm.P[5][lj][li]; q.G = sm.P[6][lj][li]; q.O = sm.P[7][lj][li];
-      return q;
-    };
-    // primitives by global cell index: the LDS tile, which always covers in-tile faces
-    auto tileP = [&](int gi, int gj) { return P_at(gi - (i0 - 2), gj - (j0 - 2)); };
-    // ... and, for faces of merged cells outside the tile, the preloaded far cells
-    // (slot from the window map), else global memory
-    auto anyP = [&](int gi, int gj) {
-      const int li = gi - (i0 - 2), lj = gj - (j0 - 2);
-      if (li >= 0 && li < S::PW && lj >= 0 && lj < S::PH) return P_at(li, lj);
-      const int wi = gi - (i0 - FAR_HALO), wj = gj - (j0 - FAR_HALO);
-      if (nF > 0 && wi >= 0 && wi < S::WW && wj >= 0 && wj < S::WH) {
-        const int e = sm.fmap[wj][wi];
-        if (e < nF) {
-          Prim<R> q;
-          q.r = sm.PF[0][e]; q.u = sm.PF[1][e]; q.v = sm.PF[2][e]; q.p = sm.PF[3][e];
-          q.Z = sm.PF[4][e]; q.F = sm.PF[5][e]; q.G = sm.PF[6][e]; q.O = sm.PF[7][e];
-          return q;
-        }
-      }
-      return prim_at<R>(I, Uin, gi, gj, tsel);
-    };
-    __syncthreads();
-
-    // ---- 2. each face flux once (FS_FLUX_X / FS_FLUX_R), times |area|; extra faces last
-    for (int f = tid; f < S::NFX + S::NFR + nEx; f += NT) {
-      R F[4], FY[4], FO;
-      if (f < S::NFX + S::NFR) {
-        const bool dirR = f >= S::NFX;
-        const int g = dirR ? f - S::NFX : f;
-        const int fx = dirR ? g % TX : g % (TX + 1);
-        const int fy = dirR ? g / TX : g / (TX + 1);
-        face_flux_w<R>(I, dirR, i0 + fx, j0 + fy, tileP, F, FY, FO);
-        if (dirR) {
-          for (int q = 0; q < 4; ++q) { sm.u.inv.Fr[q][fy][fx] = F[q]; sm.u.inv.Fr[4 + q][fy][fx] = FY[q]; }
-          sm.u.inv.Fr[8][fy][fx] = FO;
-        } else {
-          for (int q = 0; q < 4; ++q) { sm.u.inv.Fx[q][fy][fx] = F[q]; sm.u.inv.Fx[4 + q][fy][fx] = FY[q]; }
-          sm.u.inv.Fx[8][fy][fx] = FO;
-        }
-      } else {
-        const int e = f - S::NFX - S::NFR;
-        const uint2 ef = sm.ex[e];
-        face_flux_w<R>(I, ex_dir(ef), ex_i(ef), ex_j(ef), anyP, F, FY, FO);
-        for (int q = 0; q < 4; ++q) { exFlux[e * 16 + q] = F[q]; exFlux[e * 16 + 4 + q] = FY[q]; }
-        exFlux[e * 16 + 8] = FO;
-      }
-    }
-    __syncthreads();
-
-    // weighted inviscid flux of face f (W, E, S, N = 0..3) of member cell (mi, mj):
-    // the in-tile array for faces of the tile, the slave's extra-face slots (`slot`,
-    // advanced per face taken) for the others, or evaluated here when it has none
-    auto invFace = [&](int f, int mi_, int mj_, int& slot, R* F9) {
-      const bool dirR = f >= 2;
-      const int fi = mi_ + (f == 1 ? 1 : 0), fj = mj_ + (f == 3 ? 1 : 0);
-      const int fx = fi - i0, fy = fj - j0;
-      if (!dirR && fx >= 0 && fx <= TX && fy >= 0 && fy < TY) {
-        for (int q = 0; q < 9; ++q) F9[q] = sm.u.inv.Fx[q][fy][fx];
-      } else if (dirR && fx >= 0 && fx < TX && fy >= 0 && fy <= TY) {
-        for (int q = 0; q < 9; ++q) F9[q] = sm.u.inv.Fr[q][fy][fx];
-      } else if (slot >= 0 && slot < nEx) {
-        for (int q = 0; q < 9; ++q) F9[q] = exFlux[slot * 16 + q];
-        ++slot;
-      } else {
-        face_flux_w<R>(I, dirR, fi, fj, anyP, F9, F9 + 4, F9[8]);
-      }
-    };
-
-    // ---- 3. merged-group finite-volume update (FS_UPDATE)
-    const int tx = tid % TX, ty = tid / TX;
-    const int ci = i0 + tx, cj = j0 + ty;
-    const bool inDom = ci < Nx && cj < Nr;
-    const long long k = (long long)cj * Nx + ci;
-    const uint32_t code = inDom ? I.code[k] : C_IS_SLAVE;
-    const bool solid = (code & C_SOLID) != 0;
-    const bool host = inDom && !solid && !(code & C_IS_SLAVE);
-    R A[4], B[4];         // group state after the stage
-    R ubarM[5];           // member-specific B.w (inflow-speed average in column 0)
-    int mi[5], mj[5], nm = 0;
-    int mslot[5];         // first extra-face slot of each member (-1: faces in the tile or none)
-    bool guard = false;   // NaN guard hit
-    R rho_old = R(0);
-    float4 gg = make_float4(0.f, 0.f, 0.f, 0.f);
-    Prim<R> qS;           // state after the stage, as primitives (physics input)
-    if (host) {
-      mi[0] = ci; mj[0] = cj; mslot[0] = -1; nm = 1;
-      const uint32_t sl = code_slaves(code);
-      if (sl & 1u) { mi[nm] = ci - 1; mj[nm] = cj; ++nm; }
-      if (sl & 2u) { mi[nm] = ci + 1; mj[nm] = cj; ++nm; }
-      if (sl & 4u) { mi[nm] = ci; mj[nm] = cj - 1; ++nm; }
-      if (sl & 8u) { mi[nm] = ci; mj[nm] = cj + 1; ++nm; }
-      // slaves outside the tile take 3 consecutive extra-face slots each, in member order
-      int es = code_exslot(code) * 3;
-      for (int m = 1; m < nm; ++m) {
-        const bool inT = mi[m] >= i0 && mi[m] < i0 + TX && mj[m] >= j0 && mj[m] < j0 + TY;
-        mslot[m] = (inT || es < 0) ? -1 : es;
-        if (!inT && es >= 0) es += 3;
-      }
-      R L[4], LY[4], LO;
-      for (int q = 0; q < 4; ++q) {
-        L[q] = -(sm.u.inv.Fx[q][ty][tx + 1] - sm.u.inv.Fx[q][ty][tx]) - (sm.u.inv.Fr[q][ty + 1][tx] - sm.u.inv.Fr[q][ty][tx]);
-        LY[q] = -(sm.u.inv.Fx[4 + q][ty][tx + 1] - sm.u.inv.Fx[4 + q][ty][tx]) - (sm.u.inv.Fr[4 + q][ty + 1][tx] - sm.u.inv.Fr[4 + q][ty][tx]);
-      }
-      LO = -(sm.u.inv.Fx[8][ty][tx + 1] - sm.u.inv.Fx[8][ty][tx]) - (sm.u.inv.Fr[8][ty + 1][tx] - sm.u.inv.Fr[8][ty][tx]);
-      for (int m = 1; m < nm; ++m) {
-        // the slave's own net flux: west/south faces enter, east/north faces leave
-        R Fw[9], Fe_[9], Fs[9], Fn[9];
-        int slot = mslot[m];
-        invFace(0, mi[m], mj[m], slot, Fw);
-        invFace(1, mi[m], mj[m], slot, Fe_);
-        invFace(2, mi[m], mj[m], slot, Fs);
-        invFace(3, mi[m], mj[m], slot, Fn);
-        for (int q = 0; q < 4; ++q) {
-          L[q] += -(Fe_[q] - Fw[q]) - (Fn[q] - Fs[q]);
-          LY[q] += -(Fe_[4 + q] - Fw[4 + q]) - (Fn[4 + q] - Fs[4 + q]);
-        }
-        LO += -(Fe_[8] - Fw[8]) - (Fn[8] - Fs[8]);
-      }
-      gg = I.gg[k];
-      R a[4], b[4];
-      for (int q = 0; q < 4; ++q) { a[q] = Uin[q * Ns + k]; b[q] = Uin[(4 + q) * Ns + k]; }
-      const Prim<R> q = to_prim<R>(a[0], a[1], a[2], a[3], b[0], b[1], b[2], b[3]);
-      const R Wx = R(gg.z), Wr = R(gg.w);
-      const R wl = sqrt(Wx * Wx + Wr * Wr);
-      if (wl > R(1e-6)) {
-        const R iwl = rc(wl);
-        const R pw = wall_pressure<R>(q, q.u * (Wx * iwl) + q.v * (Wr * iwl));
-        L[1] -= Wx * pw;
-        L[2] -= Wr * pw;
-      }
-      L[2] += q.p * R(gg.y);
-      const R invV = rc(fmax(R(gg.x), R(1e-6)));
-      const R LB[4] = {LY[0], LY[1], LY[2] - q.G * LY[3], LO};
-      R Uq[4], UY[4];
-      if (STAGE == 1) {
-        for (int m = 0; m < 4; ++m) { Uq[m] = a[m] + dt * invV * L[m]; UY[m] = b[m] + dt * invV * LB[m]; }
-      } else {
-        R a0[4], b0[4];
-        for (int m = 0; m < 4; ++m) { a0[m] = U0[m * Ns + k]; b0[m] = U0[(4 + m) * Ns + k]; }
-        rho_old = a0[0];
-        for (int m = 0; m < 4; ++m) {
-          Uq[m] = R(0.5) * (a0[m] + a[m] + dt * invV * L[m]);
-          UY[m] = R(0.5) * (b0[m] + b[m] + dt * invV * LB[m]);
-        }
-      }
-      const R rho = fmax(Uq[0], R(1e-7));
-      const R iU0 = rc(fmax(Uq[0], R(1e-7)));
-      R vx = Uq[1] * iU0, vy = Uq[2] * iU0;
-      const R sp = sq(vx * vx + vy * vy);
-      if (sp > R(c.vMax)) { const R f = dv(R(c.vMax), sp); vx *= f; vy *= f; }
-      if (!(rho < R(1e30)) || !(vx < R(1e30) && vx > R(-1e30))) {
-        guard = true;
-        for (int m = 0; m < 4; ++m) { A[m] = R(c.ambCons[m]); B[m] = R(c.ambB[m]); }
-        for (int m = 0; m < nm; ++m) ubarM[m] = R(c.ambB[3]);
-      } else {
-        const R ke = R(0.5) * rho * (vx * vx + vy * vy);
-        const R Gn = clampR(UY[2], R(c.gamR[0]), R(c.gamR[1]));
-        const R E = fmax(Uq[3], R(1e-7) * Gn + ke);
-        const R Zn = clampR(UY[0], R(0), rho);
-        A[0] = rho; A[1] = rho * vx; A[2] = rho * vy; A[3] = E;
-        B[0] = Zn; B[1] = clampR(UY[1], R(0), Zn); B[2] = Gn;
-        B[3] = c.inj ? clampR(UY[3], R(0), rho) : R(0);
-        for (int m = 0; m < nm; ++m) {
-          R ub = B[3];
-          if (!c.inj) {
-            ub = R(0);
-            if (mi[m] == 0) {
-              const R u0 = Uin[7 * Ns + (long long)mj[m] * Nx];
-              const bool on = STAGE == 2 && cur[2] > R(c.uBar[2]);
-              ub = on ? fmin(u0 + fmin(dt * R(c.uBar[0]), R(1)) * (vx - u0), R(c.uBar[1])) : u0;
-            }
-          }
-          ubarM[m] = ub;
-        }
-      }
-      qS = to_prim<R>(A[0], A[1], A[2], A[3], B[0], B[1], B[2], ubarM[0]);
-    }
-
-    // ---- 4. viscous / wall / afterburning split step (FS_VCELL, FS_VFLUX, FS_SOURCE)
-    if (STAGE == 2 && PHYS && (c.visc || c.chem)) {  // per instance: a batch mixes both kinds
-      __syncthreads();  // inviscid fluxes consumed; reuse their LDS
-      for (int idx = tid; idx < S::TW * S::THh; idx += NT) {
-        const int ri = idx % S::TW, rj = idx / S::TW;
-        const int li = ri + 1, lj = rj + 1;
-        const int gi = i0 - 2 + li, gj = j0 - 2 + lj;
-        R T1[4] = {0, 0, 0, 0}, T2[4] = {0, 0, 0, 0};
-        if (gi >= 0 && gi < Nx && gj >= 0 && gj < Nr && sm.P[0][lj][li] >= R(0)) {
-          vcell<R>(P_at(li, lj), P_at(li + 1, lj), P_at(li - 1, lj), P_at(li, lj + 1), P_at(li, lj - 1), gi, gj,
-                   R(I.lt[(long long)gj * Nx + gi]), dt, c, T1, T2);
-        }
-        for (int q = 0; q < 4; ++q) { sm.u.vis.T[q][rj][ri] = T1[q]; sm.u.vis.T[4 + q][rj][ri] = T2[q]; }
-      }
-      __syncthreads();
-      // per-cell T1/T2: the LDS ring tile, else computed from primitives
-      auto anyT = [&](int gi, int gj, R* T1, R* T2) {
-        const int ri = gi - (i0 - 1), rj = gj - (j0 - 1);
-        if (ri >= 0 && ri < S::TW && rj >= 0 && rj < S::THh) {
-          for (int q = 0; q < 4; ++q) { T1[q] = sm.u.vis.T[q][rj][ri]; T2[q] = sm.u.vis.T[4 + q][rj][ri]; }
-          return;
-        }
-        vcell_at<R>(I, anyP, dt, gi, gj, T1, T2);
-      };
-      auto tileT = [&](int gi, int gj, R* T1, R* T2) {
-        const int ri = gi - (i0 - 1), rj = gj - (j0 - 1);
-        for (int q = 0; q < 4; ++q) { T1[q] = sm.u.vis.T[q][rj][ri]; T2[q] = sm.u.vis.T[4 + q][rj][ri]; }
-      };
-      if (c.visc) {
-        for (int f = tid; f < S::NFX + S::NFR + nEx; f += NT) {
-          R F[4], FY[3];
-          if (f < S::NFX + S::NFR) {
-            const bool dirR = f >= S::NFX;
-            const int g = dirR ? f - S::NFX : f;
-            const int fx = dirR ? g % TX : g % (TX + 1);
-            const int fy = dirR ? g / TX : g / (TX + 1);
-            vface_w<R>(I, dirR, i0 + fx, j0 + fy, tileP, tileT, F, FY);
-            if (dirR) {
-              for (int q = 0; q < 4; ++q) sm.u.vis.Vr[q][fy][fx] = F[q];
-              for (int q = 0; q < 3; ++q) sm.u.vis.Vr[4 + q][fy][fx] = FY[q];
-            } else {
-              for (int q = 0; q < 4; ++q) sm.u.vis.Vx[q][fy][fx] = F[q];
-              for (int q = 0; q < 3; ++q) sm.u.vis.Vx[4 + q][fy][fx] = FY[q];
-            }
-          } else {
-            const int e = f - S::NFX - S::NFR;
-            const uint2 ef = sm.ex[e];
-            vface_w<R>(I, ex_dir(ef), ex_i(ef), ex_j(ef), anyP, anyT, F, FY);
-            for (int q = 0; q < 4; ++q) exFlux[e * 16 + 9 + q] = F[q];
-            for (int q = 0; q < 3; ++q) exFlux[e * 16 + 13 + q] = FY[q];
-          }
-        }
-        __syncthreads();
-      }
-      auto visFace = [&](int f, int mi_, int mj_, int& slot, R* F7) {
-        const bool dirR = f >= 2;
-        const int fi = mi_ + (f == 1 ? 1 : 0), fj = mj_ + (f == 3 ? 1 : 0);
-        const int fx = fi - i0, fy = fj - j0;
-        if (!dirR && fx >= 0 && fx <= TX && fy >= 0 && fy < TY) {
-          for (int q = 0; q < 7; ++q) F7[q] = sm.u.vis.Vx[q][fy][fx];
-        } else if (dirR && fx >= 0 && fx < TX && fy >= 0 && fy <= TY) {
-          for (int q = 0; q < 7; ++q) F7[q] = sm.u.vis.Vr[q][fy][fx];
-        } else if (slot >= 0 && slot < nEx) {
-          for (int q = 0; q < 7; ++q) F7[q] = exFlux[slot * 16 + 9 + q];
-          ++slot;
-        } else {
-          vface_w<R>(I, dirR, fi, fj, anyP, anyT, F7, F7 + 4);
-        }
-      };
-      if (host) {
-        const R rho = qS.r, p = qS.p, G = qS.G, velx = qS.u, vely = qS.v;
-        const R V = fmax(R(gg.x), R(1e-6));
-        const R Rm = rmix(qS.Z, qS.F, qS.O, c), Tg = dv(p, rho * Rm);
-        R L[4] = {0, 0, 0, 0}, LY[3] = {0, 0, 0};
-        R T1h[4], T2h[4];
-        for (int q = 0; q < 4; ++q) { T1h[q] = sm.u.vis.T[q][ty + 1][tx + 1]; T2h[q] = sm.u.vis.T[4 + q][ty + 1][tx + 1]; }
-        if (c.visc) {
-          for (int m = 0; m < nm; ++m) {
-            R Fw[7], Fe_[7], Fs[7], Fn[7];
-            int slot = mslot[m];
-            visFace(0, mi[m], mj[m], slot, Fw);
-            visFace(1, mi[m], mj[m], slot, Fe_);
-            visFace(2, mi[m], mj[m], slot, Fs);
-            visFace(3, mi[m], mj[m], slot, Fn);
-            for (int q = 0; q < 4; ++q) L[q] += -(Fe_[q] - Fw[q]) - (Fn[q] - Fs[q]);
-            for (int q = 0; q < 3; ++q) LY[q] += -(Fe_[4 + q] - Fw[4 + q]) - (Fn[4 + q] - Fs[4 + q]);
-          }
-          // axisymmetric hoop stress (solver.js:587-588)
-          const R r = dv(V, fmax(R(gg.y), R(1e-6))), vor = dv(vely, r);
-          L[2] -= T2h[0] * (R(2) * vor - R(0.6666667f) * (T1h[0] + T1h[3] + vor)) * R(gg.y);
-          // Werner-Wengle wall shear + Reynolds-analogy heat flux (solver.js:590-607)
-          const R gx = R(gg.z), gr = R(gg.w);
-          const R wl = sqrt(gx * gx + gr * gr);
-          if (wl > R(1e-6)) {
-            const R iwl = rc(wl);
-            const R nx = gx * iwl, nr = gr * iwl;
-            const R vn = velx * nx + vely * nr;
-            const R utx = velx - vn * nx, utr = vely - vn * nr;
-            const R up = sqrt(utx * utx + utr * utr);
-            const R cp = Rm * (G + R(1));
-            const R Taw = Tg + dv(R(0.89) * up * up, R(2) * cp);
-            const R Ts = fmax(Tg + R(0.5) * (R(c.Tw) - Tg) + R(0.22) * (Taw - Tg), R(1e-3));
-            const R rs = dv(p, Rm * Ts), mus = sutherland(Ts * R(c.gas[2]), c);
-            const R dz = R(2) * clampR(R(0.5) * V * iwl, R(0.25), R(1));
-            const R nzv = dv(mus, rs * dz);
-            R tw = up <= R(69.7) * nzv ? dv(R(2) * mus * up, dz)
-                                       : rs * pow(R(7.187) * pow(nzv, R(1.142857)) + R(0.13769) * pow(nzv, R(0.142857)) * up, R(1.75));
-            const R iwdt = rc(fmax(wl * dt, R(1e-9)));
-            tw = fmin(tw, R(0.3) * rho * up * V * iwdt);
-            if (up > R(1e-6)) { const R f = dv(tw * wl, up); L[1] -= f * utx; L[2] -= f * utr; }
-            const R qw = dv(tw, fmax(up, R(1e-4))) * cp * (Taw - R(c.Tw)) * R(1.24);
-            const R qmax = R(0.3) * p * G * V * iwdt;
-            L[3] -= clampR(qw, -qmax, qmax) * wl;
-          }
-          // Zukoski separation in the boundary-layer band (solver.js:613-630)
-          if (code & C_BL) {
-            const R up = sq(velx * velx + vely * vely), Mn = dv(up, sq(dv((R(1) + rc(G)) * p, rho)));
-            if (Mn > R(1)) {
-              const R iup = rc(up);
-              const R shx = velx * iup, shy = vely * iup;
-              R pd = R(0);
-              for (int n = 4; n <= 6; n += 2)
-                for (int m = 0; m <= 1; ++m) {
-                  int Dx = ci + (int)round(R(n) * shx);
-                  int Dy = cj + (int)floor(R(n) * shy) + m;
-                  Dx = min(max(Dx, 0), Nx - 1);
-                  Dy = min(max(Dy, 0), Nr - 1);
-                  const long long kd = (long long)Dy * Nx + Dx;
-                  if (!(I.code[kd] & C_SOLID)) pd = fmax(pd, load_prim<R>(U1, Ns, kd).p);
-                }
-              const R crit = R(1) + R(0.5) * Mn;
-              const R fs = smoothstepR(crit, R(1.2) * crit, dv(pd, p)) * V * rho * up;
-              L[1] -= fs * velx;
-              L[2] -= fs * vely;
-            } else if (velx > R(0) && p < R(1.3) * R(c.chemK[3])) {
-              L[1] -= R(0.5) * V * rho * up * velx;
-              L[2] -= R(0.5) * V * rho * up * vely;
-            }
-          }
-        }
-        const R invV = dv(dt, V);
-        const R dO = invV * L[0];
-        R Uq[4] = {A[0], A[1] + invV * L[1], A[2] + invV * L[2], A[3] + invV * L[3]};
-        R UY[4] = {B[0] + invV * LY[0], B[1] + invV * LY[1], B[2] + dv(invV * LY[2], rho), B[3]};
-        if (c.inj) UY[3] += dO;
-        // afterburning / injector combustion (solver.js:639-654)
-        if (c.chem) {
-          const R Z = qS.Z, Fv = qS.F, O = qS.O, s = R(c.chemK[1]);
-          const R avail = c.inj ? fmin(Fv, (O + R(0.233) * fmax(R(1) - Z, R(0))) / s)
-                                : fmin(Fv, fmax(R(1) - Z - s * (Z - Fv), R(0)) / s);
-          if (avail > R(1e-5)) {
-            const R TK = Tg * R(c.gas[2]);
-            const R kin = R(c.chemK[2]) * p * exp(R(-9000) / TK) * (R(1) - smoothstepR(R(c.cap[0]), R(c.cap[1]), TK));
-            const R mixr = c.inj ? fmax(T2h[2], R(4) * T2h[0] / rho) : T2h[2];
-            const R burn = avail * fmin(fmin(mixr, kin) * dt, R(0.5));
-            UY[1] -= rho * burn;
-            if (c.inj) UY[3] = fmax(UY[3] - rho * burn * s, R(0));
-            Uq[3] += rho * burn * R(c.chemK[0]);
-          }
-        }
-        const R rn = fmax(Uq[0], R(1e-7));
-        const R irn = rc(rn);
-        const R vnx = Uq[1] * irn, vny = Uq[2] * irn;
-        if (!(rn < R(1e30)) || !(vnx < R(1e30) && vnx > R(-1e30))) {
-          guard = true;
-          for (int m = 0; m < 4; ++m) { A[m] = R(c.ambCons[m]); B[m] = R(c.ambB[m]); }
-          for (int m = 0; m < nm; ++m) ubarM[m] = R(c.ambB[3]);
-        } else {
-          const R Gn = clampR(UY[2], R(c.gamR[0]), R(c.gamR[1]));
-          const R Zn = clampR(UY[0], R(0), rn);
-          A[0] = rn; A[1] = rn * vnx; A[2] = rn * vny;
-          A[3] = fmax(Uq[3], R(1e-7) * Gn + R(0.5) * rn * (vnx * vnx + vny * vny));
-          B[0] = Zn; B[1] = clampR(UY[1], R(0), Zn); B[2] = Gn;
-          if (c.inj) {
-            B[3] = clampR(UY[3], R(0), rn);
-            for (int m = 0; m < nm; ++m) ubarM[m] = B[3];
-          }
-        }
-      }
-    }
-
-    // ---- 5. spray gas sources (FS_SPRAY_APPLY, spray.js:158-178), consumed and cleared
-    if (STAGE == 2 && c.spray && host) {
-      float* __restrict__ src = I.src;
-      const R s0 = R(src[k]), s1 = R(src[Ns + k]), s2 = R(src[2 * Ns + k]), s3 = R(src[3 * Ns + k]);
-      const R dm = s0 + s1;
-      if (!(dm == R(0) && s2 == R(0) && s3 == R(0))) {
-        src[k] = 0.f; src[Ns + k] = 0.f; src[2 * Ns + k] = 0.f; src[3 * Ns + k] = 0.f;
-        const R invV = rc(fmax(R(gg.x), R(1e-6)));
-        const R iA0 = rc(fmax(A[0], R(1e-7)));
-        const R velx = A[1] * iA0, vely = A[2] * iA0;
-        const R rn = A[0] + dm * invV;
-        const R mx = A[1] + s2 * invV, my = A[2] + s3 * invV;
-        R E = A[3] + (s0 * R(c.eF) + s1 * R(c.eO) + (velx * s2 + vely * s3)) * invV;
-        const R irn = rc(rn);
-        const R phi = fmin(dm * invV * irn, R(1));
-        const R Gn = mixR(B[2], R(c.Ge), phi);
-        const R vnx = mx * irn, vny = my * irn;
-        E = fmax(E, R(0.02) * A[3] + R(0.5) * rn * (vnx * vnx + vny * vny) + R(1e-7) * Gn);
-        A[0] = rn; A[1] = mx; A[2] = my; A[3] = E;
-        B[0] = B[0] + dm * invV;
-        B[1] = B[1] + s0 * invV;
-        B[2] = Gn;
-        // every member takes the host's B (spray.js:161-177)
-        const R bw = ubarM[0] + s1 * invV;
-        for (int m = 0; m < nm; ++m) ubarM[m] = bw;
-      }
-    }
-
-    // ---- 6. write the group, wave speed for the next step's CFL, residual
-    R* __restrict__ Uout = STAGE == 1 ? U1 : U0;
-    if (inDom && solid) {
-      for (int q = 0; q < 4; ++q) { Uout[q * Ns + k] = R(c.ambCons[q]); Uout[(4 + q) * Ns + k] = R(c.ambB[q]); }
-    }
-    if (host) {
-      for (int m = 0; m < nm; ++m) {
-        const long long km = (long long)mj[m] * Nx + mi[m];
-        for (int q = 0; q < 4; ++q) Uout[q * Ns + km] = A[q];
-        for (int q = 0; q < 3; ++q) Uout[(4 + q) * Ns + km] = B[q];
-        Uout[7 * Ns + km] = ubarM[m];
-      }
-      if (STAGE == 2) {
-        const Prim<R> qn = to_prim<R>(A[0], A[1], A[2], A[3], B[0], B[1], B[2], ubarM[0]);
-        wloc = (double)(fabs(qn.u) + fabs(qn.v) + R(2) * sound_speed(qn));
-        // slaves are fluid cells holding the same state
-        const double dr = (double)(A[0] - rho_old) / fmax((double)dt, 1e-30);
-        rloc = dr * dr;
-      }
-    }
-  }
-
-  if (STAGE == 2) {
-    // per-tile partials, reduced by k_finalize (no same-address atomics across workgroups)
-    const double wb = block_max<NT>(wloc, sm.red);
-    const double rb = block_sum<NT>(rloc, sm.redd);
-    if (tid == 0) I.part[lt_] = make_double2(active ? wb : 0.0, active ? rb : 0.0);
-  }
-}
-
 // Closes the step of every instance (one workgroup per instance): wave-speed max
 // and residual over its tiles, step info of the completed step (FS_STEPINFO's
 // input for the next one), spray ring head and seed (spray.js:295-296).
@@ -1029,35 +380,60 @@ __global__ void k_set_inst(DevInst* dst, DevInst src) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *dst = src;
 }
 
-void tile_shape(bool fp64, int* tx, int* ty) {
-  *tx = fp64 ? TX64 : TX32;
-  *ty = fp64 ? TY64 : TY32;
-}
 
 hipError_t launch_set_inst(DevInst* dst, const DevInst& src, hipStream_t s) {
   hipLaunchKernelGGL(k_set_inst, dim3(1), dim3(64), 0, s, dst, src);
   return hipGetLastError();
 }
 
-template <typename R, int TX, int TY>
-static hipError_t stage_t(int stage, bool phys, const DevInst* d, int ninst, int ntiles, hipStream_t s) {
-  const dim3 g(ntiles), b(TX * TY);
-  if (stage == 1) hipLaunchKernelGGL((k_stage<R, TX, TY, 1, false>), g, b, 0, s, d, ninst);
-  else if (phys) hipLaunchKernelGGL((k_stage<R, TX, TY, 2, true>), g, b, 0, s, d, ninst);
-  else hipLaunchKernelGGL((k_stage<R, TX, TY, 2, false>), g, b, 0, s, d, ninst);
-  return hipGetLastError();
+#define NZ_DECL_STAGE(name, R, TX, TY)                                                                  \
+  hipError_t stage_launch_##name(int stage, bool phys, const DevInst* d, int ninst, int ntiles, hipStream_t s, \
+                                 hipEvent_t e0, hipEvent_t e1);
+NZ_STAGE_SHAPES_F32(NZ_DECL_STAGE)
+NZ_STAGE_SHAPES_F64(NZ_DECL_STAGE)
+#undef NZ_DECL_STAGE
+
+namespace {
+struct ShapeEntry {
+  int tx, ty;
+  hipError_t (*launch)(int, bool, const DevInst*, int, int, hipStream_t, hipEvent_t, hipEvent_t);
+};
+#define NZ_SHAPE_ENTRY(name, R, TX, TY) {TX, TY, stage_launch_##name},
+const ShapeEntry kShapes32[] = {NZ_STAGE_SHAPES_F32(NZ_SHAPE_ENTRY)};
+const ShapeEntry kShapes64[] = {NZ_STAGE_SHAPES_F64(NZ_SHAPE_ENTRY)};
+#undef NZ_SHAPE_ENTRY
+}  // namespace
+
+int tile_shape_count(bool fp64) {
+  return fp64 ? (int)(sizeof(kShapes64) / sizeof(kShapes64[0])) : (int)(sizeof(kShapes32) / sizeof(kShapes32[0]));
 }
-hipError_t launch_stage(int stage, bool phys, bool fp64, const DevInst* d, int ninst, int ntiles, hipStream_t s) {
-  return fp64 ? stage_t<double, TX64, TY64>(stage, phys, d, ninst, ntiles, s)
-              : stage_t<float, TX32, TY32>(stage, phys, d, ninst, ntiles, s);
+
+bool tile_shape(bool fp64, int shape, int* tx, int* ty) {
+  if (shape < 0 || shape >= tile_shape_count(fp64)) return false;
+  const ShapeEntry& e = fp64 ? kShapes64[shape] : kShapes32[shape];
+  *tx = e.tx;
+  *ty = e.ty;
+  return true;
+}
+
+hipError_t launch_stage(int stage, bool phys, bool fp64, int shape, const DevInst* d, int ninst, int ntiles,
+                        hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
+  if (shape < 0 || shape >= tile_shape_count(fp64)) return hipErrorInvalidValue;
+  const ShapeEntry& e = fp64 ? kShapes64[shape] : kShapes32[shape];
+  return e.launch(stage, phys, d, ninst, ntiles, s, e0, e1);
 }
 hipError_t launch_finalize(const DevInst* d, int ninst, hipStream_t s) {
   hipLaunchKernelGGL(k_finalize, dim3(ninst), dim3(256), 0, s, d, ninst);
   return hipGetLastError();
 }
-hipError_t launch_spray(bool fp64, const DevInst* d, int ninst, int nblk, hipStream_t s) {
-  if (fp64) hipLaunchKernelGGL((k_spray<double>), dim3(nblk), dim3(256), 0, s, d, ninst);
-  else hipLaunchKernelGGL((k_spray<float>), dim3(nblk), dim3(256), 0, s, d, ninst);
+hipError_t launch_spray(bool fp64, const DevInst* d, int ninst, int nblk, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
+  if (e0 && e1) {
+    if (fp64) hipExtLaunchKernelGGL((k_spray<double>), dim3(nblk), dim3(256), 0, s, e0, e1, 0, d, ninst);
+    else hipExtLaunchKernelGGL((k_spray<float>), dim3(nblk), dim3(256), 0, s, e0, e1, 0, d, ninst);
+  } else {
+    if (fp64) hipLaunchKernelGGL((k_spray<double>), dim3(nblk), dim3(256), 0, s, d, ninst);
+    else hipLaunchKernelGGL((k_spray<float>), dim3(nblk), dim3(256), 0, s, d, ninst);
+  }
   return hipGetLastError();
 }
 hipError_t launch_cfl_init(bool fp64, const DevInst& I, hipStream_t s) {

@@ -27,6 +27,7 @@ struct Inst {
   size_t rsz = 4;
   int Nx = 0, Nr = 0;
   long long Ns = 0;
+  int shape = 0;                 // tile shape of the stage kernels (stage_shapes.h)
   int TX = 64, TY = 4, tilesX = 0, tilesY = 0;
   void* U0 = nullptr;
   void* U1 = nullptr;
@@ -44,6 +45,7 @@ struct Inst {
   float* wallrow = nullptr;
   double* dint = nullptr;
   std::vector<uint8_t> mask;
+  std::vector<uint32_t> codeH;   // host copy of the code words (tile-shape changes rebuild from it)
   std::vector<float> jWall;
   bool haveGeo = false, haveTab = false;
   int NP = 65536;
@@ -55,7 +57,9 @@ struct Inst {
 
 // device bytes an instance of this size needs (admission by HBM budget)
 size_t inst_bytes(int Nx, int Nr, int prec, int NP);
-int inst_alloc(Inst& I, int Nx, int Nr, int prec, nzk::InstState* st);
+int inst_alloc(Inst& I, int Nx, int Nr, int prec, nzk::InstState* st, int shape = 0);
+// switch the tile shape: per-tile lists and partials are rebuilt for it
+int inst_set_shape(Inst& I, int shape);
 void inst_free(Inst& I);
 int inst_set_geometry(Inst& I, const uint8_t* mask, const float* geoA, const float* geoG, const float* geoM,
                       const float* jWall);

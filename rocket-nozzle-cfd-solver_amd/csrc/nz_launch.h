@@ -6,15 +6,17 @@
 
 namespace nzk {
 
-// tile shapes of the fused stage kernels (cells per workgroup: TX wide in x, TY in r)
-constexpr int TX32 = 64, TY32 = 4;   // FP32: one wave64 per grid row
-constexpr int TX64 = 32, TY64 = 4;   // FP64: half-width tiles keep LDS under 64 KB
-
-void tile_shape(bool fp64, int* tx, int* ty);
+// tile shapes of the fused stage kernels (TX cells wide in x, TY in r; one thread
+// per cell), listed per precision in stage_shapes.h; shape 0 is the default
+int tile_shape_count(bool fp64);
+bool tile_shape(bool fp64, int shape, int* tx, int* ty);
 hipError_t launch_set_inst(DevInst* dst, const DevInst& src, hipStream_t s);
-hipError_t launch_stage(int stage, bool phys, bool fp64, const DevInst* d, int ninst, int ntiles, hipStream_t s);
+// with events e0/e1, the kernel's own start and end are recorded into them
+hipError_t launch_stage(int stage, bool phys, bool fp64, int shape, const DevInst* d, int ninst, int ntiles,
+                        hipStream_t s, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
 hipError_t launch_finalize(const DevInst* d, int ninst, hipStream_t s);
-hipError_t launch_spray(bool fp64, const DevInst* d, int ninst, int nblk, hipStream_t s);
+hipError_t launch_spray(bool fp64, const DevInst* d, int ninst, int nblk, hipStream_t s, hipEvent_t e0 = nullptr,
+                        hipEvent_t e1 = nullptr);
 hipError_t launch_cfl_init(bool fp64, const DevInst& I, hipStream_t s);
 hipError_t launch_rescale(bool fp64, const DevInst& I, float fp, float fT, float du, hipStream_t s);
 hipError_t launch_wall_row(bool fp64, const DevInst& I, const float* jw, float* out, hipStream_t s);

@@ -201,16 +201,17 @@ int nz_set_state(nz_solver* s, const float* A, const float* B) {
   return 0;
 }
 
+// ev (NULL or 6 events): the start / end of stage 1, the spray and stage 2 kernels
 static int enqueue_step(nz_solver* s, bool phys, bool spray, hipEvent_t* ev) {
   const bool fp64 = s->prec == 64;
   const int ntiles = s->I.tilesX * s->I.tilesY;
-  if (ev) HIPCHK(hipEventRecord(ev[0], s->stream), "event");
-  HIPCHK(nzk::launch_stage(1, false, fp64, s->dinst, 1, ntiles, s->stream), "stage 1");
-  if (ev) HIPCHK(hipEventRecord(ev[1], s->stream), "event");
-  if (spray) HIPCHK(nzk::launch_spray(fp64, s->dinst, 1, s->h.npblk, s->stream), "spray");
-  if (ev) HIPCHK(hipEventRecord(ev[2], s->stream), "event");
-  HIPCHK(nzk::launch_stage(2, phys, fp64, s->dinst, 1, ntiles, s->stream), "stage 2");
-  if (ev) HIPCHK(hipEventRecord(ev[3], s->stream), "event");
+  HIPCHK(nzk::launch_stage(1, false, fp64, s->I.shape, s->dinst, 1, ntiles, s->stream, ev ? ev[0] : nullptr,
+                           ev ? ev[1] : nullptr), "stage 1");
+  if (spray)
+    HIPCHK(nzk::launch_spray(fp64, s->dinst, 1, s->h.npblk, s->stream, ev ? ev[2] : nullptr, ev ? ev[3] : nullptr),
+           "spray");
+  HIPCHK(nzk::launch_stage(2, phys, fp64, s->I.shape, s->dinst, 1, ntiles, s->stream, ev ? ev[4] : nullptr,
+                           ev ? ev[5] : nullptr), "stage 2");
   HIPCHK(nzk::launch_finalize(s->dinst, 1, s->stream), "finalize");
   return 0;
 }
@@ -237,26 +238,144 @@ int nz_step_timed(nz_solver* s, int n, const nz_params* P, double* ms) {
   if (int rc = apply_params(s, P)) return rc;
   const bool phys = P->visc || P->chem;
   const bool spray = s->h.c.spray != 0;
-  std::vector<hipEvent_t> ev((size_t)n * 4);
+  // per step: the kernels' own start / end (hipExtLaunchKernel events), and the span
+  std::vector<hipEvent_t> ev((size_t)n * 6 + 2);
   for (auto& e : ev) HIPCHK(hipEventCreate(&e), "event create");
+  hipEvent_t* span = &ev[(size_t)n * 6];
   int rc = 0;
-  for (int k = 0; k < n && !rc; ++k) rc = enqueue_step(s, phys, spray, &ev[(size_t)k * 4]);
+  if (hipEventRecord(span[0], s->stream) != hipSuccess) rc = nz_fail(-10, "event");
+  for (int k = 0; k < n && !rc; ++k) rc = enqueue_step(s, phys, spray, &ev[(size_t)k * 6]);
+  if (!rc && hipEventRecord(span[1], s->stream) != hipSuccess) rc = nz_fail(-10, "event");
   HIPCHK(hipStreamSynchronize(s->stream), "nz_step_timed sync");
   double t[3] = {0, 0, 0};
   float all = 0.f;
   if (!rc) {
     for (int k = 0; k < n; ++k)
       for (int q = 0; q < 3; ++q) {
+        if (q == 1 && !spray) continue;
         float e = 0.f;
-        if (hipEventElapsedTime(&e, ev[k * 4 + q], ev[k * 4 + q + 1]) != hipSuccess) rc = nz_fail(-10, "event elapsed");
+        if (hipEventElapsedTime(&e, ev[(size_t)k * 6 + 2 * q], ev[(size_t)k * 6 + 2 * q + 1]) != hipSuccess)
+          rc = nz_fail(-10, "event elapsed");
         t[q] += e;
       }
-    if (hipEventElapsedTime(&all, ev[0], ev[(size_t)n * 4 - 1]) != hipSuccess) rc = nz_fail(-10, "event elapsed");
+    if (hipEventElapsedTime(&all, span[0], span[1]) != hipSuccess) rc = nz_fail(-10, "event elapsed");
   }
   for (auto& e : ev) (void)hipEventDestroy(e);
   if (rc) return rc;
   ms[0] = t[0]; ms[1] = t[1]; ms[2] = t[2]; ms[3] = all;
   s->stepCount += n;
+  return 0;
+}
+
+int nz_tile_shapes(int precision, int* tx, int* ty, int cap) {
+  if (precision != 32 && precision != 64) return nz_fail(-1, "nz_tile_shapes: precision must be 32 or 64");
+  const int n = nzk::tile_shape_count(precision == 64);
+  for (int k = 0; k < n && k < cap; ++k) {
+    int a = 0, b = 0;
+    nzk::tile_shape(precision == 64, k, &a, &b);
+    if (tx) tx[k] = a;
+    if (ty) ty[k] = b;
+  }
+  return n;
+}
+
+int nz_set_tile_shape(nz_solver* s, int shape) {
+  if (int rc = check_solver(s, "nz_set_tile_shape")) return rc;
+  HIPCHK(hipStreamSynchronize(s->stream), "nz_set_tile_shape: sync");
+  if (int rc = nz::inst_set_shape(s->I, shape)) return rc;
+  return send_inst(s);
+}
+
+int nz_get_tile_shape(const nz_solver* s, int* shape) {
+  if (!s || !shape) return nz_fail(-1, "nz_get_tile_shape: null argument");
+  *shape = s->I.shape;
+  return 0;
+}
+
+namespace {
+// device copy of everything a step changes, for trial runs that must leave no trace
+struct Snapshot {
+  void* U0 = nullptr;
+  void* U1 = nullptr;
+  float* parc = nullptr;
+  float* src = nullptr;
+  InstState st{};
+  ~Snapshot() {
+    if (U0) (void)hipFree(U0);
+    if (U1) (void)hipFree(U1);
+    if (parc) (void)hipFree(parc);
+    if (src) (void)hipFree(src);
+  }
+};
+int snap_take(nz_solver* s, Snapshot& z) {
+  const nz::Inst& I = s->I;
+  const size_t us = (size_t)I.Ns * 8 * I.rsz, ps = (size_t)I.NP * 8 * 4, ss = (size_t)I.Ns * 16;
+  HIPCHK(hipMalloc(&z.U0, us), "snapshot");
+  HIPCHK(hipMalloc(&z.U1, us), "snapshot");
+  HIPCHK(hipMalloc(&z.parc, ps), "snapshot");
+  HIPCHK(hipMalloc(&z.src, ss), "snapshot");
+  HIPCHK(hipMemcpyAsync(z.U0, I.U0, us, hipMemcpyDeviceToDevice, s->stream), "snapshot");
+  HIPCHK(hipMemcpyAsync(z.U1, I.U1, us, hipMemcpyDeviceToDevice, s->stream), "snapshot");
+  HIPCHK(hipMemcpyAsync(z.parc, I.parc, ps, hipMemcpyDeviceToDevice, s->stream), "snapshot");
+  HIPCHK(hipMemcpyAsync(z.src, I.src, ss, hipMemcpyDeviceToDevice, s->stream), "snapshot");
+  HIPCHK(hipMemcpyAsync(&z.st, I.st, sizeof(InstState), hipMemcpyDeviceToHost, s->stream), "snapshot");
+  HIPCHK(hipStreamSynchronize(s->stream), "snapshot");
+  return 0;
+}
+int snap_restore(nz_solver* s, const Snapshot& z) {
+  const nz::Inst& I = s->I;
+  const size_t us = (size_t)I.Ns * 8 * I.rsz, ps = (size_t)I.NP * 8 * 4, ss = (size_t)I.Ns * 16;
+  HIPCHK(hipMemcpyAsync(I.U0, z.U0, us, hipMemcpyDeviceToDevice, s->stream), "restore");
+  HIPCHK(hipMemcpyAsync(I.U1, z.U1, us, hipMemcpyDeviceToDevice, s->stream), "restore");
+  HIPCHK(hipMemcpyAsync(I.parc, z.parc, ps, hipMemcpyDeviceToDevice, s->stream), "restore");
+  HIPCHK(hipMemcpyAsync(I.src, z.src, ss, hipMemcpyDeviceToDevice, s->stream), "restore");
+  HIPCHK(hipMemcpyAsync(I.st, &z.st, sizeof(InstState), hipMemcpyHostToDevice, s->stream), "restore");
+  HIPCHK(hipStreamSynchronize(s->stream), "restore");
+  return 0;
+}
+}  // namespace
+
+int nz_autotune(nz_solver* s, const nz_params* P, int steps, int* best, double* us_per_step) {
+  if (int rc = check_solver(s, "nz_autotune")) return rc;
+  if (!P) return nz_fail(-1, "nz_autotune: null params");
+  if (steps < 1) return nz_fail(-1, "nz_autotune: step count must be positive");
+  if (!s->I.haveGeo || !s->haveState) return nz_fail(-4, "nz_autotune: setGeometry() and setState() first");
+  nz_read_async_cancel(s);
+  HIPCHK(hipStreamSynchronize(s->stream), "nz_autotune: sync");
+  Snapshot z;
+  if (int rc = snap_take(s, z)) return rc;
+  const int n = nzk::tile_shape_count(s->prec == 64), start = s->I.shape;
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0), "event");
+  if (hipEventCreate(&e1) != hipSuccess) { (void)hipEventDestroy(e0); return nz_fail(-10, "event"); }
+  int pick = start, rc = 0;
+  double bestUs = 1e300;
+  for (int k = 0; k < n && !rc; ++k) {
+    if ((rc = nz_set_tile_shape(s, k))) break;
+    if ((rc = apply_params(s, P))) break;
+    const bool phys = P->visc || P->chem, spray = s->h.c.spray != 0;
+    // one untimed step loads the shape's code object
+    if ((rc = enqueue_step(s, phys, spray, nullptr)) || (rc = snap_restore(s, z))) break;
+    if (hipEventRecord(e0, s->stream) != hipSuccess) { rc = nz_fail(-10, "event"); break; }
+    for (int q = 0; q < steps && !rc; ++q) rc = enqueue_step(s, phys, spray, nullptr);
+    if (rc) break;
+    float ms = 0.f;
+    if (hipEventRecord(e1, s->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+        hipEventElapsedTime(&ms, e0, e1) != hipSuccess) { rc = nz_fail(-10, "autotune timing"); break; }
+    const double us = ms * 1e3 / steps;
+    if (us_per_step) us_per_step[k] = us;
+    if (us < bestUs) { bestUs = us; pick = k; }
+    rc = snap_restore(s, z);
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc) {
+    (void)nz_set_tile_shape(s, start);
+    (void)snap_restore(s, z);
+    return rc;
+  }
+  if (int r = nz_set_tile_shape(s, pick)) return r;
+  if (best) *best = pick;
   return 0;
 }
 

@@ -140,6 +140,10 @@ SIGNATURES = {
     "nz_step_timed": (C.c_int, [vp, C.c_int, P(NzParams), dp]),
     "nz_rescale_ambient": (C.c_int, [vp, C.c_float, C.c_float, C.c_float, P(NzParams)]),
     "nz_draw_wall_row": (C.c_int, [vp]),
+    "nz_tile_shapes": (C.c_int, [C.c_int, P(C.c_int), P(C.c_int), C.c_int]),
+    "nz_set_tile_shape": (C.c_int, [vp, C.c_int]),
+    "nz_get_tile_shape": (C.c_int, [vp, P(C.c_int)]),
+    "nz_autotune": (C.c_int, [vp, P(NzParams), C.c_int, P(C.c_int), dp]),
     "nz_read_rect": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, fp]),
     "nz_read_step_info": (C.c_int, [vp, fp]),
     "nz_read_state_f64": (C.c_int, [vp, dp, dp]),
@@ -166,6 +170,7 @@ SIGNATURES = {
     "nz_batch_release": (C.c_int, [vp, C.c_int64]),
     "nz_batch_set_graph": (C.c_int, [vp, C.c_int]),
     "nz_batch_memory": (C.c_int, [vp, P(C.c_uint64), P(C.c_uint64)]),
+    "nz_batch_set_tile_shape": (C.c_int, [vp, C.c_int]),
     "nz_batch_finished": (C.c_int, [vp, P(C.c_int64), C.c_int]),
     "nz_config_default": (None, [P(NzConfig)]),
     "nz_config_preset": (C.c_int, [P(NzConfig), C.c_char_p]),

@@ -22,6 +22,14 @@ CPP_SOURCES = ["setup.cpp", "errors.cpp"]
 DEVICE_FLAGS = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
 
+def stage_shapes():
+    """(name, type, TX, TY) of every entry of csrc/stage_shapes.h, in order."""
+    import re
+    text = open(os.path.join(CSRC, "stage_shapes.h")).read()
+    return [(m.group(1), m.group(2), int(m.group(3)), int(m.group(4)))
+            for m in re.finditer(r"X\((\w+),\s*(float|double),\s*(\d+),\s*(\d+)\)", text)]
+
+
 def _run(cmd, cwd=None):
     r = subprocess.run(cmd, cwd=cwd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
     if r.returncode != 0:
@@ -40,6 +48,14 @@ def build_lib(jobs=8):
         obj = os.path.join(obj_dir, src + ".o")
         cmd = [HIPCC, f"--offload-arch={ARCH}", "-c", *common, *DEVICE_FLAGS, "-munsafe-fp-atomics",
                os.path.join(CSRC, src), "-o", obj]
+        procs.append((cmd, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)))
+        objs.append(obj)
+    # one object per tile shape of the stage kernels
+    for name, typ, tx, ty in stage_shapes():
+        obj = os.path.join(obj_dir, f"stage_{name}.o")
+        cmd = [HIPCC, f"--offload-arch={ARCH}", "-c", *common, *DEVICE_FLAGS, "-munsafe-fp-atomics",
+               f"-DNZ_STAGE_NAME={name}", f"-DNZ_STAGE_R={typ}", f"-DNZ_STAGE_TX={tx}", f"-DNZ_STAGE_TY={ty}",
+               os.path.join(CSRC, "stage_inst.hip"), "-o", obj]
         procs.append((cmd, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)))
         objs.append(obj)
     for src in CPP_SOURCES:

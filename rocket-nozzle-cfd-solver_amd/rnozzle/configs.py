@@ -67,6 +67,53 @@ def kerolox_injector(Nx=768, **over):
     return apply(cfg, over)
 
 
+def batch_instance(rng, Nx=None):
+    """One solve of the batched set: a conical nozzle with every range drawn from
+    `rng` (numpy Generator). Grid width a multiple of 64 in [128, 1024] with
+    Nr = 3 Nx / 8; area ratio log-uniform in [2, 40]; divergent half-angle
+    uniform in [10, 20] deg; chamber pressure log-uniform in [5, 200] bar;
+    chamber temperature uniform in [300, 3600] K; gamma uniform in [1.15, 1.4];
+    ambient pressure log-uniform in [0.05, 1] bar at 288.15 K. Throat diameter
+    20 mm, contraction ratio 5, chamber length 1.5 Rc. Below 1000 K the gas is
+    a cold gas (Mw 28, no afterburning); hotter gas has Mw 22 and afterburns
+    with the 'custom' propellant's heat release. Viscous model on, wall at
+    min(800 K, T0); the domain follows the plume-length rule (fitDomain)."""
+    if Nx is None:
+        Nx = 64 * int(rng.integers(2, 17))
+    eps = math.exp(rng.uniform(math.log(2), math.log(40)))
+    half = rng.uniform(10, 20)
+    p0 = math.exp(rng.uniform(math.log(5), math.log(200))) * 1e5
+    T0 = rng.uniform(300, 3600)
+    gam = rng.uniform(1.15, 1.4)
+    pa = math.exp(rng.uniform(math.log(0.05), math.log(1.0))) * 1e5
+    hot = T0 >= 1000
+    Dt = 0.020
+    cfg = default_config(prop="custom" if hot else "coldgas", Tc=T0, gamma=gam, Mw=22.0 if hot else 28.0,
+                         qAb=3.0 if hot else 0.0, sAir=1.0, Dt=Dt, De=Dt * math.sqrt(eps), type="cone",
+                         coneAngle=half, CR=5, chamberLen=1.5, Pc=p0, Nx=Nx, aspect=3 / 8, ambP=pa, ambT=288.15,
+                         Tw=min(800.0, T0), viscous=1, afterburn=1 if hot else 0)
+    cfg.domainL = fit_domain(cfg, pa)
+    return cfg
+
+
+def injector_instance(rng):
+    """One spray request of the serving mix: 384 x 128 kerolox injector, 16,384
+    parcels, element type uniform over showerhead / coaxial / impinging / pintle,
+    mixture ratio uniform in [2, 3.5]; chamber and nozzle of the reference's
+    injector presets (injImpinge, physics.js:246-251)."""
+    kind = ["shower", "coax", "impinge", "pintle"][int(rng.integers(0, 4))]
+    cfg = default_config(prop="kerolox", Tc=3670, gamma=1.22, Mw=22.0, qAb=6.8, sAir=1.8, Pc=70e5, Dt=0.030,
+                         De=0.085, type="bell", bellPct=80, CR=5, chamberLen=6, Nx=384, aspect=128 / 384,
+                         injMode=1, injType=kind, injRings=4 if kind != "pintle" else 1, injDP=20,
+                         injD=1.2 if kind != "pintle" else 1.0, injAngle=30, OF=rng.uniform(2.0, 3.5))
+    return cfg
+
+
+INJECTOR_PARCELS = 16384
+BATCH_T_END = 3e-3       # s simulated
+BATCH_RES_TOL = 1e-6     # residual stop
+
+
 def apply(cfg, over):
     from .sim_setup import apply_config
     return apply_config(cfg, **over)

@@ -61,6 +61,14 @@ class GPUSpray:
         return PA, PB
 
 
+def tile_shapes(precision=32):
+    """[(TX, TY)] of the stage kernels' tile shapes at this precision; index = shape id."""
+    n = lib().nz_tile_shapes(int(precision), None, None, 0)
+    tx, ty = (C.c_int * n)(), (C.c_int * n)()
+    lib().nz_tile_shapes(int(precision), tx, ty, n)
+    return list(zip(tx[:n], ty[:n]))
+
+
 class GPUSolver:
     """GPUSolver (solver.js:677-1076) on one MI355X (HIP stream per solver)."""
 
@@ -125,6 +133,24 @@ class GPUSolver:
 
     def rescaleAmbient(self, fp, fT, du, P):
         check(lib().nz_rescale_ambient(self._h, float(fp), float(fT), float(du), C.byref(P)), "rescaleAmbient")
+
+    # ---- tile shapes of the stage kernels and the autotuner (no reference counterpart)
+    def setTileShape(self, shape):
+        check(lib().nz_set_tile_shape(self._h, int(shape)), "setTileShape")
+
+    def tileShape(self):
+        v = C.c_int()
+        check(lib().nz_get_tile_shape(self._h, C.byref(v)), "tileShape")
+        return v.value
+
+    def autotune(self, P, steps=20):
+        """Time every tile shape from the current state (left unchanged) and keep the
+        fastest; returns (best shape, [us per step of each shape])."""
+        n = len(tile_shapes(self.precision))
+        us = (C.c_double * n)()
+        best = C.c_int()
+        check(lib().nz_autotune(self._h, C.byref(P), int(steps), C.byref(best), us), "autotune")
+        return best.value, list(us)
 
     def drawWallRow(self):
         check(lib().nz_draw_wall_row(self._h), "drawWallRow")
