@@ -8,6 +8,8 @@ Default workload: hot methalox bell, 768 x 256, FP32, full viscous model
 (`rnozzle.configs.hot_bell`). Prints one JSON line.
 
   python bench.py [--steps K] [--warmup W] [--workload hot_bell|cold_cone|kerolox_injector]
+  python bench.py --workload batch [--instances 64]   (batched solves/s vs one at a time)
+  python bench.py --workload serve [--requests 64]    (Poisson arrivals, time-to-converged)
 """
 import argparse
 import ctypes as C
@@ -33,6 +35,11 @@ STAGE1_BYTES = 92
 STAGE2_BYTES = 128
 STAGE2_SPRAY_EXTRA = 32
 PARCEL_BYTES = 64 + 32 + 16   # parcel read+write, gas gather, 4-float deposit
+
+# HBM-side bytes per launch of the stage-2 kernel measured with rocprofv3 FETCH_SIZE and
+# WRITE_SIZE (scaled by a 256 MiB copy: x2.0 and x1.0), for (workload, grid, tile shape);
+# profiles/hot_bell_768x256_counters.md holds the runs
+MEASURED_TRAFFIC = {("hot_bell", "768x256", 0): 48.51e6}
 
 
 def cpu_model():
@@ -227,6 +234,7 @@ def main():
     ap.add_argument("--nx", type=int, default=0, help="override the grid width (other sizes are parity cases)")
     ap.add_argument("--set", action="append", default=[], help="config override key=value (e.g. wallModel=stair)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="keep the default tile shape")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--instances", type=int, default=64, help="batch: solves in the set")
     ap.add_argument("--seed", type=int, default=20240501)
@@ -274,6 +282,13 @@ def main():
     spray = s.injector is not None
     g.step(args.warmup, s.P)
     g.sync()
+    # tile shape: the autotuner times every shape from the current state (restored after
+    # each trial) and keeps the fastest; outside the timed region
+    from rnozzle.solver import tile_shapes
+    tune_us = None
+    if not args.no_tune:
+        _, tune_us = g.autotune(s.P, steps=20)
+    shape = g.tileShape()
     # timed region: K steps enqueued back to back on the solver's stream
     t0 = time.perf_counter()
     g.step(args.steps, s.P)
@@ -293,9 +308,12 @@ def main():
     t1 = ms[0] / ktimed * 1e-3
     b2 = cells * (STAGE2_BYTES + (STAGE2_SPRAY_EXTRA if spray else 0))
     b1 = cells * STAGE1_BYTES
-    roof = {"bound": "hbm", "kernel": "k_stage<2> (flux + update + split step + CFL, fused)",
+    tx, ty = tile_shapes(32)[shape]
+    roof = {"bound": "hbm", "kernel": f"k_stage<float,{tx},{ty},2,{'true' if s.P.visc or s.P.chem else 'false'}> "
+                                      "(flux + update + split step + CFL partials, fused)",
             "achieved": b2 / t2 / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": b2 / t2 / 1e9 / HBM_PEAK_GBS,
-            "traffic": None, "bytes_per_cell": STAGE2_BYTES + (STAGE2_SPRAY_EXTRA if spray else 0),
+            "traffic": MEASURED_TRAFFIC.get((args.workload, f"{s.Nx}x{s.Nr}", shape)) if not over else None,
+            "bytes_per_cell": STAGE2_BYTES + (STAGE2_SPRAY_EXTRA if spray else 0),
             "avg_us": t2 * 1e6,
             "stage1": {"achieved": b1 / t1 / 1e9, "frac": b1 / t1 / 1e9 / HBM_PEAK_GBS, "avg_us": t1 * 1e6,
                        "bytes_per_cell": STAGE1_BYTES}}
@@ -305,7 +323,10 @@ def main():
         "vs_baseline": None, "dtype": "fp32", "data": "synthetic (setup from the workload's geometry)",
         "config": {"workload": args.workload, "overrides": over, "grid": f"{s.Nx}x{s.Nr}", "cells": cells,
                    "merged_cells": int(s.info.nMerged), "physics": bool(s.P.visc),
-                   "afterburning": bool(s.P.chem), "spray": spray, "cfl": round(float(s.P.cfl), 3)},
+                   "afterburning": bool(s.P.chem), "spray": spray, "cfl": round(float(s.P.cfl), 3),
+                   "tile": "%dx%d" % tile_shapes(32)[shape],
+                   "autotune_us_per_step": None if tune_us is None else
+                   {"%dx%d" % t: round(u, 2) for t, u in zip(tile_shapes(32), tune_us)}},
         "roofline": roof,
         "kernel_ms": {"steps": ktimed, "stage1": ms[0], "spray": ms[1], "stage2": ms[2], "events_span": ms[3]},
         "sim": {"t_ms": float(si[2]) * s.tScale * 1e3, "dt": float(si[0]), "Pc_bar": integ["Pc"] / 1e5,

@@ -154,10 +154,11 @@ int nz_resize(nz_solver* s, int Nx, int Nr) {
   if (Nx < 2 || Nr < 2 || (long long)Nx * Nr > (1LL << 28)) return nz_fail(-1, "nz_resize: grid size out of range");
   nz_read_async_cancel(s);
   HIPCHK(hipStreamSynchronize(s->stream), "nz_resize: sync");
-  const int NP = s->I.NP;
+  // the parcel count and the tile shape carry over a resize
+  const int NP = s->I.NP, shape = s->I.shape;
   free_buffers(s);
   s->I.NP = NP;
-  if (int rc = nz::inst_alloc(s->I, Nx, Nr, s->prec, nullptr)) return rc;
+  if (int rc = nz::inst_alloc(s->I, Nx, Nr, s->prec, nullptr, shape)) return rc;
   HIPCHK(hipMalloc(&s->dinst, sizeof(DevInst)), "hipMalloc instance");
   s->stepCount = 0;
   std::memset(&s->h, 0, sizeof(s->h));
