@@ -104,6 +104,19 @@ def plain_one_at_a_time(items):
     return total, cu
 
 
+def tune_shape(setup, args):
+    """Tile shape for a batch: the autotuner's pick on its largest solve (0 with --no-tune)."""
+    if args.no_tune:
+        return 0
+    from rnozzle.solver import GPUSolver, load_setup
+    g = GPUSolver()
+    load_setup(g, setup)
+    g.step(50, setup.P)
+    best, _ = g.autotune(setup.P, steps=20)
+    g.close()
+    return best
+
+
 def bench_batch(args):
     """Batched solve set: `--instances` nozzles drawn from `--seed`, each to a residual
     of 1e-6 or 3 ms simulated, all resident and advancing in the same launches;
@@ -116,7 +129,9 @@ def bench_batch(args):
     t0 = time.perf_counter()
     setups = [SimSetup(configs.batch_instance(rng)) for _ in range(args.instances)]
     t_setup = time.perf_counter() - t0
+    shape = tune_shape(max(setups, key=lambda s: s.Nx * s.Nr), args)
     b = Batch(max_resident=max(1, args.instances), budget_bytes=int(args.budget_gb * (1 << 30)))
+    b.set_tile_shape(shape)
     ids = [b.submit(s, t_end_s=configs.BATCH_T_END, res_tol=configs.BATCH_RES_TOL, min_steps=100) for s in setups]
     t0 = time.perf_counter()
     runs = 0
@@ -139,6 +154,7 @@ def bench_batch(args):
         "vs_baseline": None, "dtype": "fp32", "data": f"synthetic: {len(ids)} nozzles drawn from seed {args.seed}",
         "config": {"workload": f"batch{len(ids)}", "instances": len(ids), "seed": args.seed,
                    "t_end_s": configs.BATCH_T_END, "res_tol": configs.BATCH_RES_TOL, "chunk_steps": args.chunk,
+                   "tile_shape": shape,
                    "grids": sorted({f"{s.Nx}x{s.Nr}" for s in setups})},
         "wall_s": wall, "cell_updates_per_s": cu / wall, "setup_build_s": t_setup, "stops": stops,
         "latency_s": {"p50": pct(lat, 50), "p99": pct(lat, 99), "max": max(lat)},
@@ -167,7 +183,9 @@ def bench_serve(args):
         inj = rng.uniform() < 0.3
         cfg = configs.injector_instance(rng) if inj else configs.batch_instance(rng)
         reqs.append((float(arrivals[k]), SimSetup(cfg), inj))
+    shape = tune_shape(max((r[1] for r in reqs), key=lambda s: s.Nx * s.Nr), args)
     b = Batch(max_resident=args.requests, budget_bytes=int(args.budget_gb * (1 << 30)))
+    b.set_tile_shape(shape)
     t0 = time.perf_counter()
     m0 = time.monotonic()
     arrival_of = {}

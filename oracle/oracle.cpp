@@ -1091,6 +1091,18 @@ void oracle_read_parcels(void* p, float* PA, float* PB) {
   if (h->prec == 64) { copy_out(PA, h->d.PA[h->d.spCur], h->d.PA[0].size()); copy_out(PB, h->d.PB[h->d.spCur], h->d.PB[0].size()); }
   else { copy_out(PA, h->f.PA[h->f.spCur], h->f.PA[0].size()); copy_out(PB, h->f.PB[h->f.spCur], h->f.PB[0].size()); }
 }
+// place parcel k (a = x, r, u, v; b = d, m, type, x_impinge), for single-droplet tests
+void oracle_set_parcel(void* p, int k, const float* a, const float* b) {
+  Handle* h = static_cast<Handle*>(p);
+  auto put = [&](auto& o) {
+    if (k < 0 || k >= o.NP) return;
+    for (int q = 0; q < 4; ++q) {
+      o.PA[o.spCur][(size_t)k * 4 + q] = a[q];
+      o.PB[o.spCur][(size_t)k * 4 + q] = b[q];
+    }
+  };
+  if (h->prec == 64) put(h->d); else put(h->f);
+}
 // pieces of one pass, for closed-form tests of the restatement itself
 int oracle_pcg(uint32_t v) { return (int)Oracle<float>::pcg(v); }
 }

@@ -49,6 +49,7 @@ def lib():
             "oracle_read_t2": (None, [vp, fp]),
             "oracle_wall_row": (None, [vp, fp]),
             "oracle_read_parcels": (None, [vp, fp, fp]),
+            "oracle_set_parcel": (None, [vp, C.c_int, fp, fp]),
             "oracle_pcg": (C.c_int, [C.c_uint32]),
         }
         for n, (r, a) in sig.items():
@@ -127,6 +128,11 @@ class Oracle:
         o = np.zeros(self.Nx * 4, np.float32)
         lib().oracle_wall_row(self._h, _f(o))
         return o
+
+    def set_parcel(self, k, a, b):
+        a = np.ascontiguousarray(a, np.float32)
+        b = np.ascontiguousarray(b, np.float32)
+        lib().oracle_set_parcel(self._h, k, _f(a), _f(b))
 
     def parcels(self, n):
         PA = np.zeros(n * 4, np.float32)

@@ -89,6 +89,9 @@ class Batch:
         check(lib().nz_batch_release(self._h, jid), "nz_batch_release")
         self._setups.pop(jid, None)
 
+    def set_tile_shape(self, shape):
+        check(lib().nz_batch_set_tile_shape(self._h, int(shape)), "nz_batch_set_tile_shape")
+
     def memory(self):
         u, b = C.c_uint64(), C.c_uint64()
         check(lib().nz_batch_memory(self._h, C.byref(u), C.byref(b)), "nz_batch_memory")

@@ -230,3 +230,244 @@ def test_cold_cone_choked_mass_flow():
         o.step(250, s.P)
         samples.append(mdot())
     assert np.mean(samples) == pytest.approx(s.info.thMdot, rel=0.025)
+
+
+# ---------------------------------------------------------------- viscous split step
+MU0_ISA = 1.7894e-5   # dynamic viscosity of air at 288.15 K, U.S. Standard Atmosphere 1976 (kg/(m s))
+
+
+def viscous_params(rho=1.0, p=1.0, muK=400.0):
+    """Uniform-gas parameters with the viscous split step on, no eddy viscosity
+    (length scale 0), no chamber floor, no afterburning; gas temperature = p/rho x 288.15 K."""
+    P = uniform_params(rho=rho, p=p)
+    P.visc = 1
+    P.chem = 0
+    P.gas[:] = [1.0, 1.0, 288.15, muK]
+    P.chamberNu[:] = [0.0, 0.0]
+    P.Tw = 1.0
+    return P
+
+
+def test_axisymmetric_couette_viscous_flux():
+    """u = a r, v = 0, uniform p and rho in an axisymmetric channel. The viscous momentum
+    and energy equations give d(rho u)/dt = (1/r) d(r mu du/dr)/dr = mu a / r and
+    dE/dt = (1/r) d(r u mu du/dr)/dr = 2 mu a^2, and the finite-volume form with r-weighted
+    face areas reproduces both exactly. The inviscid stages leave this flow unchanged
+    (equal pressures, zero normal velocity on every face). mu at p/rho = 1 is the Sutherland
+    viscosity of air at 288.15 K, 1.7894e-5 kg/(m s) (U.S. Standard Atmosphere), times muK."""
+    # a = 2^-9 keeps u, rho u and E exact in float32, so p is exactly 1 in every cell
+    Nx, Nr, a, muK = 40, 24, 2.0 ** -9, 400.0
+    mask, geoA, geoG, geoM = box_geometry(Nx, Nr)
+    r = (np.arange(Nr) + 0.5)[:, None] * np.ones((1, Nx))
+    rho, p = np.ones((Nr, Nx)), np.ones((Nr, Nx))
+    A, B = state_from_prims(rho, a * r, np.zeros_like(rho), p)
+    P = viscous_params(muK=muK)
+    o = orc.Oracle(Nx, Nr, 64)
+    o.set_geometry(mask, geoA, geoG, geoM)
+    o.set_state(A, B)
+    S0 = o.state_f64()[0].reshape(Nr, Nx, 4)
+    o.step(1, P)
+    dt = float(o.step_info()[0])
+    S1 = o.state_f64()[0].reshape(Nr, Nx, 4)
+    mu = float(np.float32(muK)) * MU0_ISA
+    jj = np.arange(3, Nr - 4)            # clear of the axis and the top wall
+    ii = slice(12, Nx - 12)              # clear of the end walls' disturbance
+    dmom = (S1[jj, ii, 1] - S0[jj, ii, 1])
+    exp_mom = dt * mu * a / (jj + 0.5)
+    np.testing.assert_allclose(dmom, exp_mom[:, None] * np.ones_like(dmom), rtol=1e-4)
+    dE = S1[jj, ii, 3] - S0[jj, ii, 3]
+    np.testing.assert_allclose(dE, 2 * dt * mu * a * a * np.ones_like(dE), rtol=1e-4)
+    # density and radial momentum untouched
+    assert np.max(np.abs(S1[jj, ii, 0] - 1.0)) < 1e-12
+    assert np.max(np.abs(S1[jj, ii, 2])) < 1e-12
+
+
+def werner_wengle_tau(up, mu, rho, dz, A=8.3, B=1.0 / 7.0):
+    """Wall shear of the Werner-Wengle power law u+ = A y+^B integrated over a first cell
+    of height dz (Werner & Wengle 1991): linear below the crossover u_c = nu/(2 dz) A^(2/(1-B)),
+    rho [ (1-B)/2 A^((1+B)/(1-B)) (nu/dz)^(1+B) + (1+B)/A (nu/dz)^B u ]^(2/(1+B)) above."""
+    nu = mu / rho
+    uc = nu / (2 * dz) * A ** (2 / (1 - B))
+    if up <= uc:
+        return 2 * mu * up / dz, uc
+    return rho * ((1 - B) / 2 * A ** ((1 + B) / (1 - B)) * (nu / dz) ** (1 + B)
+                  + (1 + B) / A * (nu / dz) ** B * up) ** (2 / (1 + B)), uc
+
+
+def test_werner_wengle_wall_shear():
+    """Plug flow u = U along the top wall of an axisymmetric channel: the wall row feels only
+    the wall-model shear (no velocity gradients in the interior, slip-wall inviscid flux).
+    Its momentum loss in one step gives tau_w, compared with the Werner-Wengle law at the
+    Eckert reference temperature T* = T + 0.5 (Tw - T) + 0.22 (Taw - T), Taw = T + Pr^(1/3) U^2 / (2 cp),
+    Pr = 0.7, with Sutherland's air viscosity. The linear branch agrees to 1e-5. Above the
+    crossover the reference writes the constants rounded (7.187 for (1-B)/2 A^((1+B)/(1-B)) =
+    7.2020, 69.7 for A^(2/(1-B))/2 = 69.74), which moves tau_w by up to 0.2 %: bar 3e-3."""
+    Nx, Nr, muK = 40, 24, 40.0
+    mask, geoA, geoG, geoM = box_geometry(Nx, Nr)
+    jw = Nr - 2
+    V, wl = jw + 0.5, float(Nr - 1)
+    dz = 2 * min(max(0.5 * V / wl, 0.25), 1.0)
+    P = viscous_params(muK=muK)
+    G, cp = 2.5, 3.5                      # gamma 1.4, R = 1: cp = R (G + 1)
+    seen = set()
+    for k in (13, 26, 50, 53, 110, 420):  # U = k / 1024: rho u and E exact in float32
+        U = k / 1024.0
+        rho = np.ones((Nr, Nx))
+        A, B = state_from_prims(rho, np.full((Nr, Nx), U), np.zeros((Nr, Nx)), np.ones((Nr, Nx)))
+        o = orc.Oracle(Nx, Nr, 64)
+        o.set_geometry(mask, geoA, geoG, geoM)
+        o.set_state(A, B)
+        S0 = o.state_f64()[0].reshape(Nr, Nx, 4)
+        o.step(1, P)
+        dt = float(o.step_info()[0])
+        S1 = o.state_f64()[0].reshape(Nr, Nx, 4)
+        tau = -(S1[jw, 20, 1] - S0[jw, 20, 1]) * V / (dt * wl)
+        # interior rows feel nothing
+        assert np.max(np.abs(S1[3:jw - 2, 12:28, 1] - S0[3:jw - 2, 12:28, 1])) < 1e-12
+        Tg = 1.0
+        Taw = Tg + 0.7 ** (1 / 3) * U * U / (2 * cp)
+        Ts = Tg + 0.5 * (float(np.float32(P.Tw)) - Tg) + 0.22 * (Taw - Tg)
+        TK = Ts * 288.15
+        mus = float(np.float32(muK)) * 1.458e-6 * TK ** 1.5 / (TK + 110.4)
+        want, uc = werner_wengle_tau(U, mus, 1.0 / Ts, dz)
+        branch = "linear" if U <= uc else "power"
+        seen.add(branch)
+        assert tau == pytest.approx(want, rel=1e-5 if branch == "linear" else 3e-3), (U, uc, branch, tau, want)
+    assert seen == {"linear", "power"}
+
+
+# ---------------------------------------------------------------- single droplet
+def droplet_run(rho_g, p_g, a0, b0, fuel, dx, steps):
+    """One fuel parcel in a closed box of gas at rest (the spray's ring spawns only
+    massless parcels elsewhere). Returns the parcel's (PA, PB) after each step and dts."""
+    Nx, Nr, NP = 40, 24, 64
+    mask, geoA, geoG, geoM = box_geometry(Nx, Nr)
+    A, B = state_from_prims(np.full((Nr, Nx), rho_g), np.zeros((Nr, Nx)), np.zeros((Nr, Nx)), np.full((Nr, Nx), p_g))
+    P = uniform_params(rho=rho_g, p=p_g)
+    P.inj = 1
+    P.sprayMhat = 0.0                      # spawned parcels carry no mass
+    P.sprayTScale = dx / 287.6             # seconds per grid time unit
+    P.fuelVec[:] = fuel                    # (liquid density, boiling T, latent heat, surface tension), SI
+    P.oxVec[:] = fuel
+    P.gas[:] = [1.0, 1.0, 288.15, 1.0]
+    o = orc.Oracle(Nx, Nr, 64)
+    o.set_geometry(mask, geoA, geoG, geoM)
+    o.set_parcels(NP)
+    o.set_injector(np.zeros(256 * 3 * 4, np.float32))
+    o.set_state(A, B)
+    o.set_parcel(NP - 1, a0, b0)
+    out, dts = [], []
+    for _ in range(steps):
+        o.step(1, P)
+        dts.append(float(o.step_info()[0]) * float(np.float32(P.sprayTScale)))
+        PA, PB = o.parcels(NP)
+        out.append((PA.reshape(-1, 4)[NP - 1].astype(np.float64), PB.reshape(-1, 4)[NP - 1].astype(np.float64)))
+    return out, np.array(dts)
+
+
+def sutherland_air(TK):
+    return 1.458e-6 * TK ** 1.5 / (TK + 110.4)
+
+
+def test_droplet_stokes_drag_relaxation():
+    """A 2 um fuel drop moving slowly through still air (Re ~ 5e-4, evaporation off: the gas
+    is below the boiling point) relaxes as v(t) = v0 exp(-t / tau), tau = rho_l d^2 / (18 mu),
+    the Stokes response time. The Schiller-Naumann factor 1 + 0.15 Re^0.687 is below 1e-3 here."""
+    d, rho_l, v0, dx = 2e-6, 800.0, 1e-5, 1e-3
+    fuel = (rho_l, 1000.0, 1e6, 0.02)
+    out, dts = droplet_run(1.0, 1.0, (20.5, 12.5, v0, 0.0), (d, 1e-15, 0.0, 0.0), fuel, dx, 10)
+    tau = rho_l * d * d / (18 * sutherland_air(288.15))
+    t = np.cumsum(dts)
+    v = np.array([a[2] for a, _ in out])
+    rate = -np.log(v / np.float32(v0)) / t
+    np.testing.assert_allclose(rate, 1 / tau, rtol=2e-3)
+    assert all(b[0] == pytest.approx(d, rel=1e-6) for _, b in out)   # no evaporation
+    assert v[-1] / v0 < 0.6                                              # decayed measurably
+
+
+def test_droplet_d2_law_evaporation():
+    """A 20 um fuel drop at rest in air at 1152.6 K: d^2 falls linearly in time with the
+    Spalding rate K = 8 (mu / Pr) / rho_l ln(1 + B) (Pr = 0.7, B = cp (T - T_boil) / L),
+    times the Ranz-Marshall convective factor 1 + 0.3 Pr^(1/3) Re^(1/2) (Re from the 1 mm/s
+    slip floor), and the mass follows m ~ d^3."""
+    d0, rho_l, Tb, L, dx = 20e-6, 800.0, 400.0, 3e5, 0.1
+    fuel = (rho_l, Tb, L, 0.02)
+    rho_g, p_g = 0.25, 1.0                 # T = p / (rho R) x 288.15 K = 1152.6 K
+    out, dts = droplet_run(rho_g, p_g, (20.5, 12.5, 0.0, 0.0), (d0, 1e-15, 0.0, 0.0), fuel, dx, 8)
+    TK = p_g / rho_g * 288.15
+    mu = sutherland_air(TK)
+    cp = 287.05 * 3.5                      # gamma 1.4 air, J/(kg K)
+    Bn = cp * (TK - Tb) / L
+    t = np.cumsum(dts)
+    d = np.array([b[0] for _, b in out])
+    m = np.array([b[1] for _, b in out])
+    dprev = np.concatenate([[np.float32(d0)], d[:-1]])
+    Re = rho_g * 1.225 * 1e-3 * dprev / mu
+    K = 8 * (mu / 0.7) / rho_l * np.log1p(Bn) * (1 + 0.3 * 0.7 ** (1 / 3) * np.sqrt(Re))
+    want = np.float32(d0) ** 2 - np.cumsum(K * dts)
+    np.testing.assert_allclose(d ** 2, want, rtol=3e-3)
+    assert d[-1] ** 2 < 0.7 * d0 ** 2                                   # evaporated measurably
+    np.testing.assert_allclose(m / 1e-15, (d / np.float32(d0)) ** 3, rtol=1e-5)
+
+
+# ---------------------------------------------------------------- afterburning
+def burn_run(rho, p, Z, F, k, s=1.0, q=5.0, a=2.0 ** -6, Nx=40, Nr=24):
+    """Shear flow u = a r with exhaust fraction Z and unburnt fuel F, one step with the
+    afterburning source on and one with it off. Returns (rho burn, heat, dt) per interior cell
+    from the difference (the split step's other terms are identical in both runs)."""
+    mask, geoA, geoG, geoM = box_geometry(Nx, Nr)
+    r = (np.arange(Nr) + 0.5)[:, None] * np.ones((1, Nx))
+    A, B = state_from_prims(np.full((Nr, Nx), rho), a * r, np.zeros((Nr, Nx)), np.full((Nr, Nx), p))
+    B = B.reshape(Nr, Nx, 4)
+    B[..., 0] = rho * Z
+    B[..., 1] = rho * F
+    out = []
+    for chem in (1, 0):
+        P = viscous_params(rho=rho, p=p, muK=40.0)
+        P.chem = chem
+        P.Rs[:] = [1.0, 1.0, 1.0]           # mixture gas constant independent of composition
+        P.chemK[:] = [q, s, k, p]
+        P.cap[:] = [5000.0, 6000.0]          # no high-temperature cut-off in range
+        o = orc.Oracle(Nx, Nr, 64)
+        o.set_geometry(mask, geoA, geoG, geoM)
+        o.set_state(A, B.ravel())
+        o.step(1, P)
+        dt = float(o.step_info()[0])
+        SA, SB = o.state_f64()
+        out.append((SA.reshape(Nr, Nx, 4), SB.reshape(Nr, Nx, 4)))
+    jj, ii = slice(3, Nr - 4), slice(12, Nx - 12)
+    burn = out[1][1][jj, ii, 1] - out[0][1][jj, ii, 1]        # rho F lost to burning
+    heat = out[0][0][jj, ii, 3] - out[1][0][jj, ii, 3]        # energy released
+    return burn, heat, dt
+
+
+def test_afterburning_rates():
+    """Afterburning of unburnt fuel F with the air left in the exhaust mixture. Released heat
+    equals burnt fuel times the heat of combustion. Available fuel is min(F, (1 - Z - s (Z - F)) / s)
+    (fuel- or air-limited by the stoichiometric ratio s). The rate is the smaller of the mixing
+    rate and an Arrhenius rate k p exp(-Ta / T): where mixing limits, it equals the resolved
+    strain rate |du/dr| = a of this shear flow; where chemistry limits, it is linear in p and
+    its Arrhenius plot gives Ta = 9000 K."""
+    a = 2.0 ** -6
+    # mixing-limited (fast chemistry at 1152.6 K), fuel-limited availability F = 0.1
+    burn, heat, dt = burn_run(0.25, 1.0, Z=0.3, F=0.1, k=1e4, a=a)
+    np.testing.assert_allclose(heat, 5.0 * burn, rtol=1e-9)
+    np.testing.assert_allclose(burn / (0.25 * 0.1 * dt), a, rtol=1e-4)
+    # air-limited availability: (1 - 0.6 - 2 (0.6 - 0.5)) / 2 = 0.1 < F = 0.5
+    burn2, _, dt2 = burn_run(0.25, 1.0, Z=0.6, F=0.5, k=1e4, s=2.0, a=a)
+    np.testing.assert_allclose(burn2 / (0.25 * 0.1 * dt2), a, rtol=1e-4)
+    # no air left: nothing burns
+    burn3, _, _ = burn_run(0.25, 1.0, Z=0.9, F=0.5, k=1e4, a=a)
+    assert np.max(np.abs(burn3)) < 1e-15
+    # kinetics-limited: rate = k p exp(-Ta / T), linear in p at fixed T
+    rates = {}
+    for rho, p in ((0.25, 1.0), (0.5, 2.0), (0.3125, 1.0)):
+        b, _, d = burn_run(rho, p, Z=0.3, F=0.1, k=1.0, a=a)
+        rates[(rho, p)] = float(np.mean(b / (rho * 0.1 * d)))
+        assert np.ptp(b / (rho * 0.1 * d)) < 1e-9 * rates[(rho, p)] + 1e-18
+        assert rates[(rho, p)] < a
+    assert rates[(0.5, 2.0)] / rates[(0.25, 1.0)] == pytest.approx(2.0, rel=1e-9)
+    T1, T2 = 1.0 / 0.25 * 288.15, 1.0 / 0.3125 * 288.15
+    Ta = np.log(rates[(0.25, 1.0)] / rates[(0.3125, 1.0)]) / (1 / T2 - 1 / T1)
+    assert Ta == pytest.approx(9000.0, rel=1e-5)
+    assert rates[(0.25, 1.0)] == pytest.approx(1.0 * 1.0 * np.exp(-9000.0 / T1), rel=1e-6)
