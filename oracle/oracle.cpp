@@ -24,6 +24,24 @@
 //   GPUSpray.step / reset       spray.js:242-297
 // Templated on the arithmetic type: float reproduces the reference's FP32
 // arithmetic, double is the FP64 error reference. OpenMP over rows.
+//
+// How each pass is pinned (tests/test_oracle_closed_form.py, tests/test_setup_golden.py):
+//   prim / flux / update    exact Riemann solutions (Sod, Toro test 2), closed-box
+//                           conservation, free-stream preservation on cut cells
+//   CFL / step info         brute force over the definition
+//   inlet + whole step      choked mass flow against Pc At / c*
+//   vcell / vflux           axisymmetric Couette: momentum mu a / r and energy 2 mu a^2
+//                           exactly, with Sutherland's air viscosity (ISA table value)
+//   source: wall model      Werner-Wengle law at the Eckert reference temperature, both
+//                           branches
+//   source: afterburning    heat = fuel burnt x heat of combustion; mixing limit = shear
+//                           rate; Arrhenius limit linear in p with Ta = 9000 K;
+//                           fuel- / air-limited availability
+//   source: separation      parity unpinned (no closed form for the Zukoski probe)
+//   spray update            PCG hash (permutation, uniformity), Stokes drag relaxation,
+//                           d2-law evaporation with the Spalding number, mass balance;
+//                           break-up, impingement fan and wall bounces parity unpinned
+//   rescale, wall row       parity unpinned beyond the GPU comparison
 #include <omp.h>
 
 #include <algorithm>
