@@ -170,8 +170,10 @@ def bench_batch(args):
 
 def bench_serve(args):
     """Serving: Poisson arrivals (rate `--rate`/s) of `--requests` requests, 70 %
-    nozzles as in the batched set, 30 % 384x128 spray injectors with 16,384 parcels;
-    admitted by memory budget as others finish. Latency = completion - arrival."""
+    nozzles as in the batched set, 30 % 384x128 spray injectors with 16,384 parcels.
+    Arrived requests wait on the host; at most `--max-active` are in the runner at once
+    (admitted there by memory budget), picked cheapest-first by estimated cell-steps
+    (sjf) or in arrival order (fifo). Latency = completion - arrival."""
     import numpy as np
     from rnozzle import configs
     from rnozzle.batch import Batch
@@ -186,24 +188,36 @@ def bench_serve(args):
     shape = tune_shape(max((r[1] for r in reqs), key=lambda s: s.Nx * s.Nr), args)
     b = Batch(max_resident=args.requests, budget_bytes=int(args.budget_gb * (1 << 30)))
     b.set_tile_shape(shape)
+    def work(s):
+        # estimated cell-steps to the simulated horizon: cells x horizon / (CFL dt at the wave-speed floor)
+        return s.Nx * s.Nr * configs.BATCH_T_END / s.tScale * float(s.P.wFloor) / float(s.P.cfl)
+
     t0 = time.perf_counter()
     m0 = time.monotonic()
     arrival_of = {}
     nxt = 0
+    waiting = []   # arrived, not yet handed to the runner
     done = []
     while True:
         now = time.perf_counter() - t0
         while nxt < len(reqs) and reqs[nxt][0] <= now:
-            a, s, inj = reqs[nxt]
+            waiting.append(reqs[nxt])
+            nxt += 1
+        # admission: keep at most --max-active solves in the runner, cheapest first (sjf) or in order
+        if args.policy == "sjf":
+            waiting.sort(key=lambda r: work(r[1]))
+        r, q = b.run(0)
+        while waiting and r + q < args.max_active:
+            a, s, inj = waiting.pop(0)
             jid = b.submit(s, t_end_s=configs.BATCH_T_END, res_tol=configs.BATCH_RES_TOL, min_steps=100,
                            parcels=configs.INJECTOR_PARCELS if inj else None)
             arrival_of[jid] = (a, inj)
-            nxt += 1
+            q += 1
         r, q = b.run(args.chunk)
         done += b.finished()
-        if nxt == len(reqs) and r == 0 and q == 0:
+        if nxt == len(reqs) and not waiting and r == 0 and q == 0:
             break
-        if r == 0 and q == 0 and nxt < len(reqs):
+        if r == 0 and q == 0 and not waiting and nxt < len(reqs):
             time.sleep(max(0.0, reqs[nxt][0] - (time.perf_counter() - t0)))
     wall = time.perf_counter() - t0
     # the runner's clock is the host steady clock (CLOCK_MONOTONIC), as time.monotonic()
@@ -221,7 +235,8 @@ def bench_serve(args):
         "data": f"synthetic: {args.requests} Poisson arrivals at {args.rate}/s from seed {args.seed}",
         "config": {"workload": f"serve{args.requests}", "rate_per_s": args.rate, "requests": args.requests,
                    "injector_share": 0.3, "parcels": configs.INJECTOR_PARCELS, "t_end_s": configs.BATCH_T_END,
-                   "res_tol": configs.BATCH_RES_TOL, "chunk_steps": args.chunk},
+                   "res_tol": configs.BATCH_RES_TOL, "chunk_steps": args.chunk, "policy": args.policy,
+                   "max_active": args.max_active, "tile_shape": shape},
         "wall_s": wall,
         "time_to_converged_s": {"p50": pct(lat, 50), "p99": pct(lat, 99),
                                 "nozzle_p50": pct(lat_noz, 50), "injector_p50": pct(lat_inj, 50)},
@@ -261,6 +276,8 @@ def main():
     ap.add_argument("--budget-gb", type=float, default=64.0)
     ap.add_argument("--rate", type=float, default=200.0, help="serve: mean arrivals per second")
     ap.add_argument("--requests", type=int, default=64)
+    ap.add_argument("--policy", default="sjf", choices=["sjf", "fifo"], help="serve: admission order")
+    ap.add_argument("--max-active", type=int, default=16, help="serve: solves in the runner at once")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="run only a device copy of MIB MiB (profiler counter calibration)")
     args = ap.parse_args()
@@ -302,7 +319,7 @@ def main():
     g.sync()
     # tile shape: the autotuner times every shape from the current state (restored after
     # each trial) and keeps the fastest; outside the timed region
-    from rnozzle.solver import tile_shapes
+    from rnozzle.solver import shape_name, tile_shapes
     tune_us = None
     if not args.no_tune:
         _, tune_us = g.autotune(s.P, steps=20)
@@ -342,9 +359,9 @@ def main():
         "config": {"workload": args.workload, "overrides": over, "grid": f"{s.Nx}x{s.Nr}", "cells": cells,
                    "merged_cells": int(s.info.nMerged), "physics": bool(s.P.visc),
                    "afterburning": bool(s.P.chem), "spray": spray, "cfl": round(float(s.P.cfl), 3),
-                   "tile": "%dx%d" % tile_shapes(32)[shape],
+                   "tile": shape_name(shape),
                    "autotune_us_per_step": None if tune_us is None else
-                   {"%dx%d" % t: round(u, 2) for t, u in zip(tile_shapes(32), tune_us)}},
+                   {shape_name(k): round(u, 2) for k, u in enumerate(tune_us)}},
         "roofline": roof,
         "kernel_ms": {"steps": ktimed, "stage1": ms[0], "spray": ms[1], "stage2": ms[2], "events_span": ms[3]},
         "sim": {"t_ms": float(si[2]) * s.tScale * 1e3, "dt": float(si[0]), "Pc_bar": integ["Pc"] / 1e5,

@@ -119,11 +119,12 @@ int nz_step_timed(nz_solver* s, int n, const nz_params* P, double* ms);
 /* GPUSolver.rescaleAmbient(fp, fT, du, P) (solver.js:998-1010) */
 int nz_rescale_ambient(nz_solver* s, float fp, float fT, float du, const nz_params* P);
 /* Tile shapes of the fused stage kernels (no counterpart in the reference, whose
- * passes are full-screen draws). nz_tile_shapes fills tx[k], ty[k] for k < cap and
- * returns how many shapes the library holds at this precision; shape 0 is the
- * default. The shape changes speed only, never results beyond FP rounding order
+ * passes are full-screen draws). nz_tile_shapes fills tx[k], ty[k] and waves[k] (the
+ * waves per SIMD the kernels without the split step are compiled to fit; 1 = no
+ * constraint) for k < cap, any pointer may be NULL, and returns how many shapes the
+ * library holds at this precision; shape 0 is the default. The shape changes speed only, never results beyond FP rounding order
  * (none: each face and cell is evaluated by the same arithmetic in every shape). */
-int nz_tile_shapes(int precision, int* tx, int* ty, int cap);
+int nz_tile_shapes(int precision, int* tx, int* ty, int* waves, int cap);
 int nz_set_tile_shape(nz_solver* s, int shape);
 int nz_get_tile_shape(const nz_solver* s, int* shape);
 /* Autotuner: times `steps` steps of every shape from the current state, restoring

@@ -386,7 +386,7 @@ hipError_t launch_set_inst(DevInst* dst, const DevInst& src, hipStream_t s) {
   return hipGetLastError();
 }
 
-#define NZ_DECL_STAGE(name, R, TX, TY)                                                                  \
+#define NZ_DECL_STAGE(name, R, TX, TY, W)                                                                  \
   hipError_t stage_launch_##name(int stage, bool phys, const DevInst* d, int ninst, int ntiles, hipStream_t s, \
                                  hipEvent_t e0, hipEvent_t e1);
 NZ_STAGE_SHAPES_F32(NZ_DECL_STAGE)
@@ -395,10 +395,10 @@ NZ_STAGE_SHAPES_F64(NZ_DECL_STAGE)
 
 namespace {
 struct ShapeEntry {
-  int tx, ty;
+  int tx, ty, w;
   hipError_t (*launch)(int, bool, const DevInst*, int, int, hipStream_t, hipEvent_t, hipEvent_t);
 };
-#define NZ_SHAPE_ENTRY(name, R, TX, TY) {TX, TY, stage_launch_##name},
+#define NZ_SHAPE_ENTRY(name, R, TX, TY, W) {TX, TY, W, stage_launch_##name},
 const ShapeEntry kShapes32[] = {NZ_STAGE_SHAPES_F32(NZ_SHAPE_ENTRY)};
 const ShapeEntry kShapes64[] = {NZ_STAGE_SHAPES_F64(NZ_SHAPE_ENTRY)};
 #undef NZ_SHAPE_ENTRY
@@ -408,11 +408,12 @@ int tile_shape_count(bool fp64) {
   return fp64 ? (int)(sizeof(kShapes64) / sizeof(kShapes64[0])) : (int)(sizeof(kShapes32) / sizeof(kShapes32[0]));
 }
 
-bool tile_shape(bool fp64, int shape, int* tx, int* ty) {
+bool tile_shape(bool fp64, int shape, int* tx, int* ty, int* waves) {
   if (shape < 0 || shape >= tile_shape_count(fp64)) return false;
   const ShapeEntry& e = fp64 ? kShapes64[shape] : kShapes32[shape];
   *tx = e.tx;
   *ty = e.ty;
+  if (waves) *waves = e.w;
   return true;
 }
 

@@ -103,6 +103,28 @@ struct DevInst {
   Consts c;
 };
 
+// Global-memory access: a uniform 64-bit base (SGPRs) plus a 32-bit per-lane byte offset
+// in the global address space, so the compiler emits global loads / stores with an SGPR
+// base instead of flat accesses through generic pointers with 64-bit lane addresses.
+// Every array indexed this way holds fewer than 2^28 elements per base.
+#define NZ_GLOBAL __attribute__((address_space(1)))
+template <class T>
+__device__ __forceinline__ T gld(const T* base, unsigned i) {
+  unsigned off = i * (unsigned)sizeof(T);
+  return *(const NZ_GLOBAL T*)((const NZ_GLOBAL char*)base + off);
+}
+__device__ __forceinline__ float4 gld(const float4* base, unsigned i) {
+  typedef float v4f __attribute__((ext_vector_type(4)));
+  unsigned off = i * 16u;
+  const v4f v = *(const NZ_GLOBAL v4f*)((const NZ_GLOBAL char*)base + off);
+  return make_float4(v.x, v.y, v.z, v.w);
+}
+template <class T>
+__device__ __forceinline__ void gst(T* base, unsigned i, T v) {
+  unsigned off = i * (unsigned)sizeof(T);
+  *(NZ_GLOBAL T*)((NZ_GLOBAL char*)base + off) = v;
+}
+
 template <typename R>
 struct Prim {
   R r, u, v, p, Z, F, G, O;
@@ -157,7 +179,9 @@ __device__ __forceinline__ Prim<R> wall_prim() {
 
 template <typename R>
 __device__ __forceinline__ Prim<R> load_prim(const R* __restrict__ U, long long Ns, long long k) {
-  return to_prim<R>(U[k], U[Ns + k], U[2 * Ns + k], U[3 * Ns + k], U[4 * Ns + k], U[5 * Ns + k], U[6 * Ns + k], U[7 * Ns + k]);
+  const unsigned i = (unsigned)k;
+  return to_prim<R>(gld(U, i), gld(U + Ns, i), gld(U + 2 * Ns, i), gld(U + 3 * Ns, i), gld(U + 4 * Ns, i),
+                    gld(U + 5 * Ns, i), gld(U + 6 * Ns, i), gld(U + 7 * Ns, i));
 }
 
 // Primitive state with boundary ghost cells at cell (i, j), i in [-2, Nx+1],
@@ -175,15 +199,15 @@ __device__ Prim<R> prim_at(const DevInst& I, const R* __restrict__ U, int i, int
   q.Z = R(0); q.F = R(0); q.G = R(c.inlet[3]); q.O = R(0);
   if (i < 0) {
     const bool inCh = R(j) + R(0.5) < R(c.RcCells);
-    if (j < Nr && (I.code[(long long)j * Nx] & C_SOLID)) return wall_prim<R>();
+    if (j < Nr && (gld(I.code, (unsigned)(j * Nx)) & C_SOLID)) return wall_prim<R>();
     if (inCh && c.inj) return wall_prim<R>();
     if (inCh && j < Nr) {
       const R f = R(c.ramp) > R(0) ? smoothstepR(R(0), R(c.ramp), t) : R(1);
       const R P0 = mixR(R(c.pStart), R(c.res[0]), f);
       const R T0 = R(c.res[1]), Rg = R(c.res[2]), G = R(c.inlet[2]), g = R(1) + R(1) / G;
       const long long k = (long long)j * Nx;
-      const R a0 = U[k], a1 = U[Ns + k];
-      const R ubar = fmax(U[7 * Ns + k], R(0));
+      const R a0 = gld(U, (unsigned)k), a1 = gld(U + Ns, (unsigned)k);
+      const R ubar = fmax(gld(U + 7 * Ns, (unsigned)k), R(0));
       const R c0 = sqrt(g * Rg * T0);
       const R ui = clampR(a1 / fmax(a0, R(1e-7)), R(0), R(0.5) * c0);
       const R ut = ubar * smoothstepR(R(1), R(2), P0 / fmax(R(c.pStart), R(1e-6)));
@@ -196,7 +220,7 @@ __device__ Prim<R> prim_at(const DevInst& I, const R* __restrict__ U, int i, int
   } else if (j >= Nr) {
     const int ci = min(i, Nx - 1), cj = Nr - 1;
     const long long k = (long long)cj * Nx + ci;
-    if (!(I.code[k] & C_SOLID)) {
+    if (!(gld(I.code, (unsigned)k) & C_SOLID)) {
       const Prim<R> tq = load_prim(U, Ns, k);
       const R cs = sound_speed(tq);
       if (tq.v >= cs) q = tq;
@@ -204,7 +228,7 @@ __device__ Prim<R> prim_at(const DevInst& I, const R* __restrict__ U, int i, int
     }
   } else if (i >= Nx) {
     const long long k = (long long)j * Nx + (Nx - 1);
-    if (!(I.code[k] & C_SOLID)) {
+    if (!(gld(I.code, (unsigned)k) & C_SOLID)) {
       const Prim<R> tq = load_prim(U, Ns, k);
       const R cs = sound_speed(tq);
       if (tq.u >= cs) q = tq;
@@ -213,7 +237,7 @@ __device__ Prim<R> prim_at(const DevInst& I, const R* __restrict__ U, int i, int
     }
   } else {
     const long long k = (long long)j * Nx + i;
-    if (I.code[k] & C_SOLID) return wall_prim<R>();
+    if (gld(I.code, (unsigned)k) & C_SOLID) return wall_prim<R>();
     q = load_prim(U, Ns, k);
   }
   if (mir) q.v = -q.v;

@@ -9,7 +9,7 @@ namespace nzk {
 // tile shapes of the fused stage kernels (TX cells wide in x, TY in r; one thread
 // per cell), listed per precision in stage_shapes.h; shape 0 is the default
 int tile_shape_count(bool fp64);
-bool tile_shape(bool fp64, int shape, int* tx, int* ty);
+bool tile_shape(bool fp64, int shape, int* tx, int* ty, int* waves = nullptr);
 hipError_t launch_set_inst(DevInst* dst, const DevInst& src, hipStream_t s);
 // with events e0/e1, the kernel's own start and end are recorded into them
 hipError_t launch_stage(int stage, bool phys, bool fp64, int shape, const DevInst* d, int ninst, int ntiles,

@@ -75,7 +75,7 @@ __device__ __forceinline__ void face_flux_w(const DevInst& I, bool dirR, int fi,
   const int Nx = I.Nx, Nr = I.Nr;
   const bool inDom = dirR ? (fi >= 0 && fi < Nx && fj >= 0 && fj <= Nr) : (fi >= 0 && fi <= Nx && fj >= 0 && fj < Nr);
   if (!inDom) return;
-  const float area = dirR ? I.ar[(long long)fj * Nx + fi] : I.ax[(long long)fj * (Nx + 1) + fi];
+  const float area = dirR ? gld(I.ar, (unsigned)(fj * Nx + fi)) : gld(I.ax, (unsigned)(fj * (Nx + 1) + fi));
   if (area == 0.f || (dirR && fj == 0)) return;
   const int dx = dirR ? 0 : 1, dy = dirR ? 1 : 0, tx = dy, ty = dx;
   Prim<R> c0 = getP(fi - 2 * dx, fj - 2 * dy), c1 = getP(fi - dx, fj - dy);
@@ -112,7 +112,7 @@ __device__ __forceinline__ void vface_w(const DevInst& I, bool dirR, int fi, int
   const int Nx = I.Nx, Nr = I.Nr;
   const bool on = dirR ? (fi >= 0 && fi < Nx && fj > 0 && fj < Nr) : (fi > 0 && fi < Nx && fj >= 0 && fj < Nr);
   if (!on) return;
-  const float area = dirR ? I.ar[(long long)fj * Nx + fi] : I.ax[(long long)fj * (Nx + 1) + fi];
+  const float area = dirR ? gld(I.ar, (unsigned)(fj * Nx + fi)) : gld(I.ax, (unsigned)(fj * (Nx + 1) + fi));
   if (area == 0.f) return;
   const int k1i = dirR ? fi : fi - 1, k1j = dirR ? fj - 1 : fj;
   const Prim<R> w1 = getP(k1i, k1j), w2 = getP(fi, fj);
@@ -132,7 +132,7 @@ __device__ __forceinline__ void vcell_at(const DevInst& I, GetP getP, R dt, int 
   for (int q = 0; q < 4; ++q) { T1[q] = R(0); T2[q] = R(0); }
   const Prim<R> w0 = getP(i, j);
   if (w0.r < R(0)) return;
-  vcell<R>(w0, getP(i + 1, j), getP(i - 1, j), getP(i, j + 1), getP(i, j - 1), i, j, R(I.lt[(long long)j * I.Nx + i]), dt,
+  vcell<R>(w0, getP(i + 1, j), getP(i - 1, j), getP(i, j + 1), getP(i, j - 1), i, j, R(gld(I.lt, (unsigned)(j * I.Nx + i))), dt,
            I.c, T1, T2);
 }
 
@@ -150,22 +150,35 @@ __device__ __forceinline__ int ex_j(uint2 e) { return (int)e.y; }
 
 // ------------------------------------------------------------------------------
 template <typename R, int TX, int TY>
+struct StageInvSmem {
+  R Fx[9][TY][TX + 1];
+  R Fr[9][TY + 1][TX];
+};
+template <typename R, int TX, int TY>
+struct StageVisSmem {
+  R T[8][TY + 2][TX + 2];
+  R Vx[7][TY][TX + 1];
+  R Vr[7][TY + 1][TX];
+};
+// the viscous arrays share the inviscid flux arrays' LDS, and exist only in the kernels
+// that run the split step
+template <typename R, int TX, int TY, bool VIS>
+union StageFluxSmem {
+  StageInvSmem<R, TX, TY> inv;
+};
+template <typename R, int TX, int TY>
+union StageFluxSmem<R, TX, TY, true> {
+  StageInvSmem<R, TX, TY> inv;
+  StageVisSmem<R, TX, TY> vis;
+};
+
+template <typename R, int TX, int TY, bool VIS>
 struct StageSmem {
   static constexpr int PW = TX + 4, PH = TY + 4;   // primitive tile with 2-cell halo
   static constexpr int NFX = (TX + 1) * TY, NFR = TX * (TY + 1);
   static constexpr int TW = TX + 2, THh = TY + 2;  // T1/T2 tile with 1-cell ring
   R P[8][PH][PW];
-  union {
-    struct {
-      R Fx[9][TY][TX + 1];
-      R Fr[9][TY + 1][TX];
-    } inv;
-    struct {
-      R T[8][THh][TW];
-      R Vx[7][TY][TX + 1];
-      R Vr[7][TY + 1][TX];
-    } vis;
-  } u;
+  StageFluxSmem<R, TX, TY, VIS> u;
   static constexpr int MAXE = MAX_EXTRA_FACES;     // faces of merged cells outside the tile
   static constexpr int MAXF = MAX_FAR_CELLS;       // cells those faces reach beyond the halo
   static constexpr int WW = TX + 2 * FAR_HALO, WH = TY + 2 * FAR_HALO;
@@ -179,10 +192,12 @@ struct StageSmem {
   int last;
 };
 
-template <typename R, int TX, int TY, int STAGE, bool PHYS>
-__global__ void __launch_bounds__(TX* TY) k_stage(const DevInst* __restrict__ insts, int ninst) {
+// W: minimum waves per SIMD the compiler must fit for stage 1 and the stage 2 kernel
+// without the split step (1: no constraint); the viscous stage 2 is never constrained
+template <typename R, int TX, int TY, int STAGE, bool PHYS, int W>
+__global__ void __launch_bounds__(TX* TY, (STAGE == 2 && PHYS) ? 1 : W) k_stage(const DevInst* __restrict__ insts, int ninst) {
   constexpr int NT = TX * TY;
-  using S = StageSmem<R, TX, TY>;
+  using S = StageSmem<R, TX, TY, STAGE == 2 && PHYS>;
   __shared__ S sm;
   const int tid = threadIdx.x;
   const int t = blockIdx.x;
@@ -291,8 +306,8 @@ __global__ void __launch_bounds__(TX* TY) k_stage(const DevInst* __restrict__ in
         const int e = f - S::NFX - S::NFR;
         const uint2 ef = sm.ex[e];
         face_flux_w<R>(I, ex_dir(ef), ex_i(ef), ex_j(ef), anyP, F, FY, FO);
-        for (int q = 0; q < 4; ++q) { exFlux[e * 16 + q] = F[q]; exFlux[e * 16 + 4 + q] = FY[q]; }
-        exFlux[e * 16 + 8] = FO;
+        for (int q = 0; q < 4; ++q) { gst(exFlux, e * 16 + q, F[q]); gst(exFlux, e * 16 + 4 + q, FY[q]); }
+        gst(exFlux, e * 16 + 8, FO);
       }
     }
     __syncthreads();
@@ -309,7 +324,7 @@ __global__ void __launch_bounds__(TX* TY) k_stage(const DevInst* __restrict__ in
       } else if (dirR && fx >= 0 && fx < TX && fy >= 0 && fy <= TY) {
         for (int q = 0; q < 9; ++q) F9[q] = sm.u.inv.Fr[q][fy][fx];
       } else if (slot >= 0 && slot < nEx) {
-        for (int q = 0; q < 9; ++q) F9[q] = exFlux[slot * 16 + q];
+        for (int q = 0; q < 9; ++q) F9[q] = gld(exFlux, slot * 16 + q);
         ++slot;
       } else {
         face_flux_w<R>(I, dirR, fi, fj, anyP, F9, F9 + 4, F9[8]);
@@ -321,7 +336,7 @@ __global__ void __launch_bounds__(TX* TY) k_stage(const DevInst* __restrict__ in
     const int ci = i0 + tx, cj = j0 + ty;
     const bool inDom = ci < Nx && cj < Nr;
     const long long k = (long long)cj * Nx + ci;
-    const uint32_t code = inDom ? I.code[k] : C_IS_SLAVE;
+    const uint32_t code = inDom ? gld(I.code, (unsigned)k) : C_IS_SLAVE;
     const bool solid = (code & C_SOLID) != 0;
     const bool host = inDom && !solid && !(code & C_IS_SLAVE);
     R A[4], B[4];         // group state after the stage
@@ -366,9 +381,9 @@ __global__ void __launch_bounds__(TX* TY) k_stage(const DevInst* __restrict__ in
         }
         LO += -(Fe_[8] - Fw[8]) - (Fn[8] - Fs[8]);
       }
-      gg = I.gg[k];
+      gg = gld(I.gg, (unsigned)k);
       R a[4], b[4];
-      for (int q = 0; q < 4; ++q) { a[q] = Uin[q * Ns + k]; b[q] = Uin[(4 + q) * Ns + k]; }
+      for (int q = 0; q < 4; ++q) { a[q] = gld(Uin + q * Ns, (unsigned)k); b[q] = gld(Uin + (4 + q) * Ns, (unsigned)k); }
       const Prim<R> q = to_prim<R>(a[0], a[1], a[2], a[3], b[0], b[1], b[2], b[3]);
       const R Wx = R(gg.z), Wr = R(gg.w);
       const R wl = sqrt(Wx * Wx + Wr * Wr);
@@ -386,7 +401,7 @@ __global__ void __launch_bounds__(TX* TY) k_stage(const DevInst* __restrict__ in
         for (int m = 0; m < 4; ++m) { Uq[m] = a[m] + dt * invV * L[m]; UY[m] = b[m] + dt * invV * LB[m]; }
       } else {
         R a0[4], b0[4];
-        for (int m = 0; m < 4; ++m) { a0[m] = U0[m * Ns + k]; b0[m] = U0[(4 + m) * Ns + k]; }
+        for (int m = 0; m < 4; ++m) { a0[m] = gld(U0 + m * Ns, (unsigned)k); b0[m] = gld(U0 + (4 + m) * Ns, (unsigned)k); }
         rho_old = a0[0];
         for (int m = 0; m < 4; ++m) {
           Uq[m] = R(0.5) * (a0[m] + a[m] + dt * invV * L[m]);
@@ -415,7 +430,7 @@ __global__ void __launch_bounds__(TX* TY) k_stage(const DevInst* __restrict__ in
           if (!c.inj) {
             ub = R(0);
             if (mi[m] == 0) {
-              const R u0 = Uin[7 * Ns + (long long)mj[m] * Nx];
+              const R u0 = gld(Uin + 7 * Ns, (unsigned)(mj[m] * Nx));
               const bool on = STAGE == 2 && cur[2] > R(c.uBar[2]);
               ub = on ? fmin(u0 + fmin(dt * R(c.uBar[0]), R(1)) * (vx - u0), R(c.uBar[1])) : u0;
             }
@@ -427,7 +442,8 @@ __global__ void __launch_bounds__(TX* TY) k_stage(const DevInst* __restrict__ in
     }
 
     // ---- 4. viscous / wall / afterburning split step (FS_VCELL, FS_VFLUX, FS_SOURCE)
-    if (STAGE == 2 && PHYS && (c.visc || c.chem)) {  // per instance: a batch mixes both kinds
+    if constexpr (STAGE == 2 && PHYS) {
+    if (c.visc || c.chem) {  // per instance: a batch mixes both kinds
       __syncthreads();  // inviscid fluxes consumed; reuse their LDS
       for (int idx = tid; idx < S::TW * S::THh; idx += NT) {
         const int ri = idx % S::TW, rj = idx / S::TW;
@@ -436,7 +452,7 @@ __global__ void __launch_bounds__(TX* TY) k_stage(const DevInst* __restrict__ in
         R T1[4] = {0, 0, 0, 0}, T2[4] = {0, 0, 0, 0};
         if (gi >= 0 && gi < Nx && gj >= 0 && gj < Nr && sm.P[0][lj][li] >= R(0)) {
           vcell<R>(P_at(li, lj), P_at(li + 1, lj), P_at(li - 1, lj), P_at(li, lj + 1), P_at(li, lj - 1), gi, gj,
-                   R(I.lt[(long long)gj * Nx + gi]), dt, c, T1, T2);
+                   R(gld(I.lt, (unsigned)(gj * Nx + gi))), dt, c, T1, T2);
         }
         for (int q = 0; q < 4; ++q) { sm.u.vis.T[q][rj][ri] = T1[q]; sm.u.vis.T[4 + q][rj][ri] = T2[q]; }
       }
@@ -474,8 +490,8 @@ __global__ void __launch_bounds__(TX* TY) k_stage(const DevInst* __restrict__ in
             const int e = f - S::NFX - S::NFR;
             const uint2 ef = sm.ex[e];
             vface_w<R>(I, ex_dir(ef), ex_i(ef), ex_j(ef), anyP, anyT, F, FY);
-            for (int q = 0; q < 4; ++q) exFlux[e * 16 + 9 + q] = F[q];
-            for (int q = 0; q < 3; ++q) exFlux[e * 16 + 13 + q] = FY[q];
+            for (int q = 0; q < 4; ++q) gst(exFlux, e * 16 + 9 + q, F[q]);
+            for (int q = 0; q < 3; ++q) gst(exFlux, e * 16 + 13 + q, FY[q]);
           }
         }
         __syncthreads();
@@ -489,7 +505,7 @@ __global__ void __launch_bounds__(TX* TY) k_stage(const DevInst* __restrict__ in
         } else if (dirR && fx >= 0 && fx < TX && fy >= 0 && fy <= TY) {
           for (int q = 0; q < 7; ++q) F7[q] = sm.u.vis.Vr[q][fy][fx];
         } else if (slot >= 0 && slot < nEx) {
-          for (int q = 0; q < 7; ++q) F7[q] = exFlux[slot * 16 + 9 + q];
+          for (int q = 0; q < 7; ++q) F7[q] = gld(exFlux, slot * 16 + 9 + q);
           ++slot;
         } else {
           vface_w<R>(I, dirR, fi, fj, anyP, anyT, F7, F7 + 4);
@@ -554,7 +570,7 @@ __global__ void __launch_bounds__(TX* TY) k_stage(const DevInst* __restrict__ in
                   Dx = min(max(Dx, 0), Nx - 1);
                   Dy = min(max(Dy, 0), Nr - 1);
                   const long long kd = (long long)Dy * Nx + Dx;
-                  if (!(I.code[kd] & C_SOLID)) pd = fmax(pd, load_prim<R>(U1, Ns, kd).p);
+                  if (!(gld(I.code, (unsigned)kd) & C_SOLID)) pd = fmax(pd, load_prim<R>(U1, Ns, kd).p);
                 }
               const R crit = R(1) + R(0.5) * Mn;
               const R fs = smoothstepR(crit, R(1.2) * crit, dv(pd, p)) * V * rho * up;
@@ -606,14 +622,16 @@ __global__ void __launch_bounds__(TX* TY) k_stage(const DevInst* __restrict__ in
         }
       }
     }
+    }
 
     // ---- 5. spray gas sources (FS_SPRAY_APPLY, spray.js:158-178), consumed and cleared
     if (STAGE == 2 && c.spray && host) {
       float* __restrict__ src = I.src;
-      const R s0 = R(src[k]), s1 = R(src[Ns + k]), s2 = R(src[2 * Ns + k]), s3 = R(src[3 * Ns + k]);
+      const unsigned ku = (unsigned)k;
+      const R s0 = R(gld(src, ku)), s1 = R(gld(src + Ns, ku)), s2 = R(gld(src + 2 * Ns, ku)), s3 = R(gld(src + 3 * Ns, ku));
       const R dm = s0 + s1;
       if (!(dm == R(0) && s2 == R(0) && s3 == R(0))) {
-        src[k] = 0.f; src[Ns + k] = 0.f; src[2 * Ns + k] = 0.f; src[3 * Ns + k] = 0.f;
+        gst(src, ku, 0.f); gst(src + Ns, ku, 0.f); gst(src + 2 * Ns, ku, 0.f); gst(src + 3 * Ns, ku, 0.f);
         const R invV = rc(fmax(R(gg.x), R(1e-6)));
         const R iA0 = rc(fmax(A[0], R(1e-7)));
         const R velx = A[1] * iA0, vely = A[2] * iA0;
@@ -638,14 +656,14 @@ __global__ void __launch_bounds__(TX* TY) k_stage(const DevInst* __restrict__ in
     // ---- 6. write the group, wave speed for the next step's CFL, residual
     R* __restrict__ Uout = STAGE == 1 ? U1 : U0;
     if (inDom && solid) {
-      for (int q = 0; q < 4; ++q) { Uout[q * Ns + k] = R(c.ambCons[q]); Uout[(4 + q) * Ns + k] = R(c.ambB[q]); }
+      for (int q = 0; q < 4; ++q) { gst(Uout + q * Ns, (unsigned)k, R(c.ambCons[q])); gst(Uout + (4 + q) * Ns, (unsigned)k, R(c.ambB[q])); }
     }
     if (host) {
       for (int m = 0; m < nm; ++m) {
         const long long km = (long long)mj[m] * Nx + mi[m];
-        for (int q = 0; q < 4; ++q) Uout[q * Ns + km] = A[q];
-        for (int q = 0; q < 3; ++q) Uout[(4 + q) * Ns + km] = B[q];
-        Uout[7 * Ns + km] = ubarM[m];
+        for (int q = 0; q < 4; ++q) gst(Uout + q * Ns, (unsigned)km, A[q]);
+        for (int q = 0; q < 3; ++q) gst(Uout + (4 + q) * Ns, (unsigned)km, B[q]);
+        gst(Uout + 7 * Ns, (unsigned)km, ubarM[m]);
       }
       if (STAGE == 2) {
         const Prim<R> qn = to_prim<R>(A[0], A[1], A[2], A[3], B[0], B[1], B[2], ubarM[0]);

@@ -268,14 +268,15 @@ int nz_step_timed(nz_solver* s, int n, const nz_params* P, double* ms) {
   return 0;
 }
 
-int nz_tile_shapes(int precision, int* tx, int* ty, int cap) {
+int nz_tile_shapes(int precision, int* tx, int* ty, int* waves, int cap) {
   if (precision != 32 && precision != 64) return nz_fail(-1, "nz_tile_shapes: precision must be 32 or 64");
   const int n = nzk::tile_shape_count(precision == 64);
   for (int k = 0; k < n && k < cap; ++k) {
-    int a = 0, b = 0;
-    nzk::tile_shape(precision == 64, k, &a, &b);
+    int a = 0, b = 0, w = 1;
+    nzk::tile_shape(precision == 64, k, &a, &b, &w);
     if (tx) tx[k] = a;
     if (ty) ty[k] = b;
+    if (waves) waves[k] = w;
   }
   return n;
 }

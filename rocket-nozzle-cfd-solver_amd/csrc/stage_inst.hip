@@ -1,12 +1,13 @@
 // stage_inst.hip — one tile shape's stage kernels and their launcher. Compiled
 // once per entry of stage_shapes.h with -DNZ_STAGE_NAME=<name> -DNZ_STAGE_R=<type>
-// -DNZ_STAGE_TX=<tx> -DNZ_STAGE_TY=<ty>, so the shapes build in parallel.
+// -DNZ_STAGE_TX=<tx> -DNZ_STAGE_TY=<ty> -DNZ_STAGE_W=<waves>, so the shapes build in parallel.
 #include <hip/hip_ext.h>
 
 #include "nz_stage.h"
 
-#if !defined(NZ_STAGE_NAME) || !defined(NZ_STAGE_R) || !defined(NZ_STAGE_TX) || !defined(NZ_STAGE_TY)
-#error "stage_inst.hip needs NZ_STAGE_NAME, NZ_STAGE_R, NZ_STAGE_TX, NZ_STAGE_TY"
+#if !defined(NZ_STAGE_NAME) || !defined(NZ_STAGE_R) || !defined(NZ_STAGE_TX) || !defined(NZ_STAGE_TY) || \
+    !defined(NZ_STAGE_W)
+#error "stage_inst.hip needs NZ_STAGE_NAME, NZ_STAGE_R, NZ_STAGE_TX, NZ_STAGE_TY, NZ_STAGE_W"
 #endif
 
 #define NZ_CAT2(a, b) a##b
@@ -27,11 +28,11 @@ hipError_t go(K k, dim3 g, dim3 b, hipStream_t s, hipEvent_t e0, hipEvent_t e1, 
 hipError_t NZ_CAT(stage_launch_, NZ_STAGE_NAME)(int stage, bool phys, const DevInst* d, int ninst, int ntiles,
                                                  hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
   using R = NZ_STAGE_R;
-  constexpr int TX = NZ_STAGE_TX, TY = NZ_STAGE_TY;
+  constexpr int TX = NZ_STAGE_TX, TY = NZ_STAGE_TY, W = NZ_STAGE_W;
   const dim3 g(ntiles), b(TX * TY);
-  if (stage == 1) return go(k_stage<R, TX, TY, 1, false>, g, b, s, e0, e1, d, ninst);
-  if (phys) return go(k_stage<R, TX, TY, 2, true>, g, b, s, e0, e1, d, ninst);
-  return go(k_stage<R, TX, TY, 2, false>, g, b, s, e0, e1, d, ninst);
+  if (stage == 1) return go(k_stage<R, TX, TY, 1, false, W>, g, b, s, e0, e1, d, ninst);
+  if (phys) return go(k_stage<R, TX, TY, 2, true, W>, g, b, s, e0, e1, d, ninst);
+  return go(k_stage<R, TX, TY, 2, false, W>, g, b, s, e0, e1, d, ninst);
 }
 
 }  // namespace nzk

@@ -140,7 +140,7 @@ SIGNATURES = {
     "nz_step_timed": (C.c_int, [vp, C.c_int, P(NzParams), dp]),
     "nz_rescale_ambient": (C.c_int, [vp, C.c_float, C.c_float, C.c_float, P(NzParams)]),
     "nz_draw_wall_row": (C.c_int, [vp]),
-    "nz_tile_shapes": (C.c_int, [C.c_int, P(C.c_int), P(C.c_int), C.c_int]),
+    "nz_tile_shapes": (C.c_int, [C.c_int, P(C.c_int), P(C.c_int), P(C.c_int), C.c_int]),
     "nz_set_tile_shape": (C.c_int, [vp, C.c_int]),
     "nz_get_tile_shape": (C.c_int, [vp, P(C.c_int)]),
     "nz_autotune": (C.c_int, [vp, P(NzParams), C.c_int, P(C.c_int), dp]),

@@ -23,11 +23,11 @@ DEVICE_FLAGS = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
 
 def stage_shapes():
-    """(name, type, TX, TY) of every entry of csrc/stage_shapes.h, in order."""
+    """(name, type, TX, TY, W) of every entry of csrc/stage_shapes.h, in order."""
     import re
     text = open(os.path.join(CSRC, "stage_shapes.h")).read()
-    return [(m.group(1), m.group(2), int(m.group(3)), int(m.group(4)))
-            for m in re.finditer(r"X\((\w+),\s*(float|double),\s*(\d+),\s*(\d+)\)", text)]
+    return [(m.group(1), m.group(2), int(m.group(3)), int(m.group(4)), int(m.group(5)))
+            for m in re.finditer(r"X\((\w+),\s*(float|double),\s*(\d+),\s*(\d+),\s*(\d+)\)", text)]
 
 
 def _run(cmd, cwd=None):
@@ -51,10 +51,11 @@ def build_lib(jobs=8):
         procs.append((cmd, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)))
         objs.append(obj)
     # one object per tile shape of the stage kernels
-    for name, typ, tx, ty in stage_shapes():
+    for name, typ, tx, ty, w in stage_shapes():
         obj = os.path.join(obj_dir, f"stage_{name}.o")
         cmd = [HIPCC, f"--offload-arch={ARCH}", "-c", *common, *DEVICE_FLAGS, "-munsafe-fp-atomics",
                f"-DNZ_STAGE_NAME={name}", f"-DNZ_STAGE_R={typ}", f"-DNZ_STAGE_TX={tx}", f"-DNZ_STAGE_TY={ty}",
+               f"-DNZ_STAGE_W={w}",
                os.path.join(CSRC, "stage_inst.hip"), "-o", obj]
         procs.append((cmd, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)))
         objs.append(obj)

@@ -61,12 +61,18 @@ class GPUSpray:
         return PA, PB
 
 
-def tile_shapes(precision=32):
-    """[(TX, TY)] of the stage kernels' tile shapes at this precision; index = shape id."""
-    n = lib().nz_tile_shapes(int(precision), None, None, 0)
-    tx, ty = (C.c_int * n)(), (C.c_int * n)()
-    lib().nz_tile_shapes(int(precision), tx, ty, n)
-    return list(zip(tx[:n], ty[:n]))
+def tile_shapes(precision=32, waves=False):
+    """[(TX, TY)] (or [(TX, TY, W)] with waves=True) of the stage kernels' tile shapes at
+    this precision; index = shape id."""
+    n = lib().nz_tile_shapes(int(precision), None, None, None, 0)
+    tx, ty, w = (C.c_int * n)(), (C.c_int * n)(), (C.c_int * n)()
+    lib().nz_tile_shapes(int(precision), tx, ty, w, n)
+    return list(zip(tx[:n], ty[:n], w[:n])) if waves else list(zip(tx[:n], ty[:n]))
+
+
+def shape_name(shape, precision=32):
+    tx, ty, w = tile_shapes(precision, waves=True)[shape]
+    return f"{tx}x{ty}" + (f"w{w}" if w > 1 else "")
 
 
 class GPUSolver:
