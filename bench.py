@@ -129,11 +129,11 @@ def bench_batch(args):
     t0 = time.perf_counter()
     setups = [SimSetup(configs.batch_instance(rng)) for _ in range(args.instances)]
     t_setup = time.perf_counter() - t0
-    shape = tune_shape(max(setups, key=lambda s: s.Nx * s.Nr), args)
     b = Batch(max_resident=max(1, args.instances), budget_bytes=int(args.budget_gb * (1 << 30)))
-    b.set_tile_shape(shape)
     ids = [b.submit(s, t_end_s=configs.BATCH_T_END, res_tol=configs.BATCH_RES_TOL, min_steps=100) for s in setups]
     t0 = time.perf_counter()
+    # the autotuner runs inside the timed region: its trial steps advance the solves
+    shape, tune_us = b.autotune(20) if not args.no_tune else (0, None)
     runs = 0
     while True:
         r, q = b.run(args.chunk)
@@ -154,7 +154,7 @@ def bench_batch(args):
         "vs_baseline": None, "dtype": "fp32", "data": f"synthetic: {len(ids)} nozzles drawn from seed {args.seed}",
         "config": {"workload": f"batch{len(ids)}", "instances": len(ids), "seed": args.seed,
                    "t_end_s": configs.BATCH_T_END, "res_tol": configs.BATCH_RES_TOL, "chunk_steps": args.chunk,
-                   "tile_shape": shape,
+                   "tile_shape": shape, "autotune_us_per_step": tune_us,
                    "grids": sorted({f"{s.Nx}x{s.Nr}" for s in setups})},
         "wall_s": wall, "cell_updates_per_s": cu / wall, "setup_build_s": t_setup, "stops": stops,
         "latency_s": {"p50": pct(lat, 50), "p99": pct(lat, 99), "max": max(lat)},

@@ -250,6 +250,11 @@ int nz_batch_set_graph(nz_batch* b, int on);
 int nz_batch_memory(const nz_batch* b, uint64_t* used, uint64_t* budget);
 /* tile shape of every resident and later admission (nz_tile_shapes) */
 int nz_batch_set_tile_shape(nz_batch* b, int shape);
+/* autotuner on the live batch: admits what fits, runs `steps` steps (plus one
+ * untimed) with every tile shape in turn and keeps the fastest. The solves advance
+ * during the trials (it is real work); stop tests still apply and the stopped solves
+ * retire at the next nz_batch_run. us_per_step: NULL or >= nz_tile_shapes() entries. */
+int nz_batch_autotune(nz_batch* b, int steps, int* best, double* us_per_step);
 /* ids of the jobs that finished in the last nz_batch_run; returns the count
  * (at most cap are written) */
 int nz_batch_finished(const nz_batch* b, int64_t* ids, int cap);

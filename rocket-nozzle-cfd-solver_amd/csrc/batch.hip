@@ -436,6 +436,40 @@ int nz_batch_set_tile_shape(nz_batch* b, int shape) {
   return 0;
 }
 
+int nz_batch_autotune(nz_batch* b, int steps, int* best, double* us_per_step) {
+  if (!b) return nz_fail(-1, "nz_batch_autotune: null batch");
+  if (steps < 1) return nz_fail(-1, "nz_batch_autotune: step count must be positive");
+  if (int rc = admit_queued(b)) return rc;
+  const int n = nzk::tile_shape_count(b->prec == 64);
+  if (b->res.empty()) {
+    if (best) *best = b->shape;
+    return 0;
+  }
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0), "event");
+  if (hipEventCreate(&e1) != hipSuccess) { (void)hipEventDestroy(e0); return nz_fail(-10, "event"); }
+  int pick = b->shape, rc = 0;
+  double bestUs = 1e300;
+  for (int k = 0; k < n && !rc; ++k) {
+    if ((rc = nz_batch_set_tile_shape(b, k)) || (rc = upload_table(b))) break;
+    // one untimed step loads the shape's code object; every step is real work
+    if ((rc = enqueue_steps(b, 1))) break;
+    if (hipEventRecord(e0, b->stream) != hipSuccess) { rc = nz_fail(-10, "event"); break; }
+    if ((rc = enqueue_steps(b, steps))) break;
+    float ms = 0.f;
+    if (hipEventRecord(e1, b->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+        hipEventElapsedTime(&ms, e0, e1) != hipSuccess) { rc = nz_fail(-10, "batch autotune timing"); break; }
+    const double us = ms * 1e3 / steps;
+    if (us_per_step) us_per_step[k] = us;
+    if (us < bestUs) { bestUs = us; pick = k; }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc) return rc;
+  if (best) *best = pick;
+  return nz_batch_set_tile_shape(b, pick);
+}
+
 int nz_batch_memory(const nz_batch* b, uint64_t* used, uint64_t* budget) {
   if (!b) return nz_fail(-1, "nz_batch_memory: null batch");
   if (used) *used = b->used;

@@ -171,6 +171,7 @@ SIGNATURES = {
     "nz_batch_set_graph": (C.c_int, [vp, C.c_int]),
     "nz_batch_memory": (C.c_int, [vp, P(C.c_uint64), P(C.c_uint64)]),
     "nz_batch_set_tile_shape": (C.c_int, [vp, C.c_int]),
+    "nz_batch_autotune": (C.c_int, [vp, C.c_int, P(C.c_int), dp]),
     "nz_batch_finished": (C.c_int, [vp, P(C.c_int64), C.c_int]),
     "nz_config_default": (None, [P(NzConfig)]),
     "nz_config_preset": (C.c_int, [P(NzConfig), C.c_char_p]),

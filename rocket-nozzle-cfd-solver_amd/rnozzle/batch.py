@@ -92,6 +92,15 @@ class Batch:
     def set_tile_shape(self, shape):
         check(lib().nz_batch_set_tile_shape(self._h, int(shape)), "nz_batch_set_tile_shape")
 
+    def autotune(self, steps=20):
+        """Run `steps` steps of the live batch with every tile shape and keep the fastest;
+        returns (best shape, [us per step of each shape])."""
+        n = lib().nz_tile_shapes(self.precision, None, None, None, 0)
+        us = (C.c_double * n)()
+        best = C.c_int()
+        check(lib().nz_batch_autotune(self._h, int(steps), C.byref(best), us), "nz_batch_autotune")
+        return best.value, list(us)
+
     def memory(self):
         u, b = C.c_uint64(), C.c_uint64()
         check(lib().nz_batch_memory(self._h, C.byref(u), C.byref(b)), "nz_batch_memory")

@@ -156,3 +156,22 @@ def test_rejects_oversized_job():
     b = Batch(budget_bytes=1 << 20, max_resident=2)
     with pytest.raises(Exception):
         b.submit(s, max_steps=10)
+
+
+def test_batch_autotune_keeps_results():
+    """The live-batch autotuner advances the solves with every tile shape in turn; the
+    results stay bit-identical to the solves run alone (every shape computes the same)."""
+    rng = np.random.default_rng(3)
+    setups = [SimSetup(configs.batch_instance(rng, Nx=nx)) for nx in (256, 128)] + [SimSetup(configs.hot_bell(Nx=192))]
+    steps = 100
+    b = Batch(max_resident=8)
+    ids = [b.submit(s, max_steps=steps, keep_state=True) for s in setups]
+    best, us = b.autotune(4)
+    say(f"[batch] autotune: best {best}, us/step " + " ".join(f"{u:.1f}" for u in us))
+    assert all(u > 0 for u in us) and us[best] == min(us)
+    run_to_end(b, chunk=16)
+    for s, jid in zip(setups, ids):
+        assert b.result(jid)["steps"] == steps
+        A, B = b.read_state(jid)
+        _, A1, B1 = alone(s, steps)
+        assert np.array_equal(A, A1) and np.array_equal(B, B1)
