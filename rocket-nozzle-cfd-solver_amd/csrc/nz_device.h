@@ -184,14 +184,33 @@ __device__ __forceinline__ Prim<R> load_prim(const R* __restrict__ U, long long 
                     gld(U + 5 * Ns, i), gld(U + 6 * Ns, i), gld(U + 7 * Ns, i));
 }
 
-// Primitive state with boundary ghost cells at cell (i, j), i in [-2, Nx+1],
-// j in [-2, Nr+1]: stagnation inlet with ignition ramp and absorbing face,
-// far field, outflow, axis mirror; solid cells flagged rho = -1 (solver.js:70-125).
+// Raw state and code word of the one grid cell that the ghost rules of cell (i, j)
+// read: (i, j) clamped into the grid after the axis mirror (j < 0 -> -j - 1). Loading
+// it unconditionally lets a thread issue the loads of all its cells at once.
 template <typename R>
-__device__ Prim<R> prim_at(const DevInst& I, const R* __restrict__ U, int i, int j, R t) {
+struct CellRaw {
+  R v[8];
+  uint32_t code;
+};
+template <typename R>
+__device__ __forceinline__ CellRaw<R> cell_src_load(const DevInst& I, const R* __restrict__ U, int i, int j) {
+  const int jm = j < 0 ? -j - 1 : j;
+  const int ci = min(max(i, 0), I.Nx - 1), cj = min(max(jm, 0), I.Nr - 1);
+  const unsigned k = (unsigned)(cj * I.Nx + ci);
+  CellRaw<R> s;
+  s.code = gld(I.code, k);
+  for (int q = 0; q < 8; ++q) s.v[q] = gld(U + q * I.Ns, k);
+  return s;
+}
+
+// Primitive state with boundary ghost cells at cell (i, j), i in [-2, Nx+1],
+// j in [-2, Nr+1], from its source cell: stagnation inlet with ignition ramp and
+// absorbing face, far field, outflow, axis mirror; solid cells flagged rho = -1
+// (solver.js:70-125).
+template <typename R>
+__device__ Prim<R> prim_from(const DevInst& I, const CellRaw<R>& s, int i, int j, R t) {
   const Consts& c = I.c;
   const int Nx = I.Nx, Nr = I.Nr;
-  const long long Ns = I.Ns;
   const bool mir = j < 0;
   if (mir) j = -j - 1;
   Prim<R> q;
@@ -199,15 +218,14 @@ __device__ Prim<R> prim_at(const DevInst& I, const R* __restrict__ U, int i, int
   q.Z = R(0); q.F = R(0); q.G = R(c.inlet[3]); q.O = R(0);
   if (i < 0) {
     const bool inCh = R(j) + R(0.5) < R(c.RcCells);
-    if (j < Nr && (gld(I.code, (unsigned)(j * Nx)) & C_SOLID)) return wall_prim<R>();
+    if (j < Nr && (s.code & C_SOLID)) return wall_prim<R>();
     if (inCh && c.inj) return wall_prim<R>();
     if (inCh && j < Nr) {
       const R f = R(c.ramp) > R(0) ? smoothstepR(R(0), R(c.ramp), t) : R(1);
       const R P0 = mixR(R(c.pStart), R(c.res[0]), f);
       const R T0 = R(c.res[1]), Rg = R(c.res[2]), G = R(c.inlet[2]), g = R(1) + R(1) / G;
-      const long long k = (long long)j * Nx;
-      const R a0 = gld(U, (unsigned)k), a1 = gld(U + Ns, (unsigned)k);
-      const R ubar = fmax(gld(U + 7 * Ns, (unsigned)k), R(0));
+      const R a0 = s.v[0], a1 = s.v[1];
+      const R ubar = fmax(s.v[7], R(0));
       const R c0 = sqrt(g * Rg * T0);
       const R ui = clampR(a1 / fmax(a0, R(1e-7)), R(0), R(0.5) * c0);
       const R ut = ubar * smoothstepR(R(1), R(2), P0 / fmax(R(c.pStart), R(1e-6)));
@@ -218,30 +236,31 @@ __device__ Prim<R> prim_at(const DevInst& I, const R* __restrict__ U, int i, int
       q.Z = R(1); q.F = R(1); q.G = G; q.O = R(0);
     }
   } else if (j >= Nr) {
-    const int ci = min(i, Nx - 1), cj = Nr - 1;
-    const long long k = (long long)cj * Nx + ci;
-    if (!(gld(I.code, (unsigned)k) & C_SOLID)) {
-      const Prim<R> tq = load_prim(U, Ns, k);
+    if (!(s.code & C_SOLID)) {
+      const Prim<R> tq = to_prim<R>(s.v[0], s.v[1], s.v[2], s.v[3], s.v[4], s.v[5], s.v[6], s.v[7]);
       const R cs = sound_speed(tq);
       if (tq.v >= cs) q = tq;
       else if (tq.v > R(0)) { q = tq; q.p = R(c.amb[3]); }
     }
   } else if (i >= Nx) {
-    const long long k = (long long)j * Nx + (Nx - 1);
-    if (!(gld(I.code, (unsigned)k) & C_SOLID)) {
-      const Prim<R> tq = load_prim(U, Ns, k);
+    if (!(s.code & C_SOLID)) {
+      const Prim<R> tq = to_prim<R>(s.v[0], s.v[1], s.v[2], s.v[3], s.v[4], s.v[5], s.v[6], s.v[7]);
       const R cs = sound_speed(tq);
       if (tq.u >= cs) q = tq;
       else if (tq.u > R(0)) { q = tq; q.p = R(c.amb[3]); }
       else q.u = fmax(q.u, R(0));
     }
   } else {
-    const long long k = (long long)j * Nx + i;
-    if (gld(I.code, (unsigned)k) & C_SOLID) return wall_prim<R>();
-    q = load_prim(U, Ns, k);
+    if (s.code & C_SOLID) return wall_prim<R>();
+    q = to_prim<R>(s.v[0], s.v[1], s.v[2], s.v[3], s.v[4], s.v[5], s.v[6], s.v[7]);
   }
   if (mir) q.v = -q.v;
   return q;
+}
+
+template <typename R>
+__device__ __forceinline__ Prim<R> prim_at(const DevInst& I, const R* __restrict__ U, int i, int j, R t) {
+  return prim_from<R>(I, cell_src_load<R>(I, U, i, j), i, j, t);
 }
 
 // van Leer limited slope of one component (solver.js:156-160)
@@ -325,15 +344,6 @@ __device__ __forceinline__ void face_flux(Prim<R> c0, Prim<R> c1, Prim<R> c2, Pr
     FY[2] = uR * GR; FY[3] = uR;
     return;
   }
-  const R inv = rc(SR - SL);
-  const R SLR = SL * SR;
-  const R Fh0 = (SR * FL0 - SL * FR0 + SLR * (rR - rL)) * inv;
-  const R Fh1 = (SR * FL1 - SL * FR1 + SLR * (rR * uR - rL * uL)) * inv;
-  const R Fh2 = (SR * FL2 - SL * FR2 + SLR * (rR * vR - rL * vL)) * inv;
-  const R Fh3 = (SR * FL3 - SL * FR3 + SLR * (ER - EL)) * inv;
-  const R Yh0 = (SR * mL * WL.Z - SL * mR * WR.Z + SLR * (rR * WR.Z - rL * WL.Z)) * inv;
-  const R Yh1 = (SR * mL * WL.F - SL * mR * WR.F + SLR * (rR * WR.F - rL * WL.F)) * inv;
-  const R Yh2 = (SR * mL * WL.O - SL * mR * WR.O + SLR * (rR * WR.O - rL * WL.O)) * inv;
   const R Ss = dv(pR - pL + mL * (SL - uL) - mR * (SR - uR), rL * (SL - uL) - rR * (SR - uR));
   const bool left = Ss >= R(0);
   const R rK = left ? rL : rR, uK = left ? uL : uR, vK = left ? vL : vR, pK = left ? pL : pR;
@@ -345,13 +355,33 @@ __device__ __forceinline__ void face_flux(Prim<R> c0, Prim<R> c1, Prim<R> c2, Pr
   const R kk = rK * (SK - uK) * iSKs;
   const R irK = rc(rK);
   const R Us0 = kk, Us1 = kk * Ss, Us2 = kk * vK, Us3 = kk * (EK * irK + (Ss - uK) * (Ss + dv(pK * irK, SK - uK)));
-  F[0] = mixR(FK0 + SK * (Us0 - rK), Fh0, hllw);
-  F[1] = mixR(FK1 + SK * (Us1 - mK), Fh1, hllw);
-  F[2] = mixR(FK2 + SK * (Us2 - rK * vK), Fh2, hllw);
-  F[3] = mixR(FK3 + SK * (Us3 - EK), Fh3, hllw);
-  FY[0] = mixR(mK * YK0 + SK * (kk * YK0 - rK * YK0), Yh0, hllw);
-  FY[1] = mixR(mK * YK1 + SK * (kk * YK1 - rK * YK1), Yh1, hllw);
-  FO = mixR(mK * YK2 + SK * (kk * YK2 - rK * YK2), Yh2, hllw);
+  F[0] = FK0 + SK * (Us0 - rK);
+  F[1] = FK1 + SK * (Us1 - mK);
+  F[2] = FK2 + SK * (Us2 - rK * vK);
+  F[3] = FK3 + SK * (Us3 - EK);
+  FY[0] = mK * YK0 + SK * (kk * YK0 - rK * YK0);
+  FY[1] = mK * YK1 + SK * (kk * YK1 - rK * YK1);
+  FO = mK * YK2 + SK * (kk * YK2 - rK * YK2);
+  // blend towards HLL (mix(hllc, hll, w)); with w = 0 the mix returns the HLLC flux
+  // itself, so the HLL flux is formed only where some face of the wave needs it
+  if (hllw > R(0)) {
+    const R inv = rc(SR - SL);
+    const R SLR = SL * SR;
+    const R Fh0 = (SR * FL0 - SL * FR0 + SLR * (rR - rL)) * inv;
+    const R Fh1 = (SR * FL1 - SL * FR1 + SLR * (rR * uR - rL * uL)) * inv;
+    const R Fh2 = (SR * FL2 - SL * FR2 + SLR * (rR * vR - rL * vL)) * inv;
+    const R Fh3 = (SR * FL3 - SL * FR3 + SLR * (ER - EL)) * inv;
+    const R Yh0 = (SR * mL * WL.Z - SL * mR * WR.Z + SLR * (rR * WR.Z - rL * WL.Z)) * inv;
+    const R Yh1 = (SR * mL * WL.F - SL * mR * WR.F + SLR * (rR * WR.F - rL * WL.F)) * inv;
+    const R Yh2 = (SR * mL * WL.O - SL * mR * WR.O + SLR * (rR * WR.O - rL * WL.O)) * inv;
+    F[0] = mixR(F[0], Fh0, hllw);
+    F[1] = mixR(F[1], Fh1, hllw);
+    F[2] = mixR(F[2], Fh2, hllw);
+    F[3] = mixR(F[3], Fh3, hllw);
+    FY[0] = mixR(FY[0], Yh0, hllw);
+    FY[1] = mixR(FY[1], Yh1, hllw);
+    FO = mixR(FO, Yh2, hllw);
+  }
   const R uf = uK + SK * ((SK - uK) * iSKs - R(1));
   FY[2] = uf * (left ? GL : GR);
   FY[3] = uf;

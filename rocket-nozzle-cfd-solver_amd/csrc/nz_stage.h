@@ -67,15 +67,17 @@ __device__ __forceinline__ double block_sum(double v, double* red) {
 // ------------------------------------------------------------------------------
 // One face's inviscid flux times |area| (FS_FLUX_X/R for the face, as memberFlux
 // weights it), with primitives from `getP(i, j)`: the LDS tile, or the tile with a
-// global fall-back for faces of merged cells that lie outside it.
-template <typename R, class GetP>
-__device__ __forceinline__ void face_flux_w(const DevInst& I, bool dirR, int fi, int fj, GetP getP, R* F, R* FY, R& FO) {
+// global fall-back for faces of merged cells that lie outside it; the signed area from
+// `getA(dirR, i, j)` (LDS for the tile's own faces, else global memory).
+template <typename R, class GetP, class GetA>
+__device__ __forceinline__ void face_flux_w(const DevInst& I, bool dirR, int fi, int fj, GetP getP, GetA getA, R* F,
+                                            R* FY, R& FO) {
   for (int q = 0; q < 4; ++q) { F[q] = R(0); FY[q] = R(0); }
   FO = R(0);
   const int Nx = I.Nx, Nr = I.Nr;
   const bool inDom = dirR ? (fi >= 0 && fi < Nx && fj >= 0 && fj <= Nr) : (fi >= 0 && fi <= Nx && fj >= 0 && fj < Nr);
   if (!inDom) return;
-  const float area = dirR ? gld(I.ar, (unsigned)(fj * Nx + fi)) : gld(I.ax, (unsigned)(fj * (Nx + 1) + fi));
+  const float area = getA(dirR, fi, fj);
   if (area == 0.f || (dirR && fj == 0)) return;
   const int dx = dirR ? 0 : 1, dy = dirR ? 1 : 0, tx = dy, ty = dx;
   Prim<R> c0 = getP(fi - 2 * dx, fj - 2 * dy), c1 = getP(fi - dx, fj - dy);
@@ -105,14 +107,15 @@ __device__ __forceinline__ void face_flux_w(const DevInst& I, bool dirR, int fi,
 
 // One face's viscous flux times |area| (FS_VFLUX_X/R); getT(i, j, T1, T2) gives the
 // per-cell gradients / viscosities of an interior fluid cell.
-template <typename R, class GetP, class GetT>
-__device__ __forceinline__ void vface_w(const DevInst& I, bool dirR, int fi, int fj, GetP getP, GetT getT, R* F, R* FY) {
+template <typename R, class GetP, class GetT, class GetA>
+__device__ __forceinline__ void vface_w(const DevInst& I, bool dirR, int fi, int fj, GetP getP, GetT getT, GetA getA, R* F,
+                                        R* FY) {
   for (int q = 0; q < 4; ++q) F[q] = R(0);
   for (int q = 0; q < 3; ++q) FY[q] = R(0);
   const int Nx = I.Nx, Nr = I.Nr;
   const bool on = dirR ? (fi >= 0 && fi < Nx && fj > 0 && fj < Nr) : (fi > 0 && fi < Nx && fj >= 0 && fj < Nr);
   if (!on) return;
-  const float area = dirR ? gld(I.ar, (unsigned)(fj * Nx + fi)) : gld(I.ax, (unsigned)(fj * (Nx + 1) + fi));
+  const float area = getA(dirR, fi, fj);
   if (area == 0.f) return;
   const int k1i = dirR ? fi : fi - 1, k1j = dirR ? fj - 1 : fj;
   const Prim<R> w1 = getP(k1i, k1j), w2 = getP(fi, fj);
@@ -148,6 +151,11 @@ __device__ __forceinline__ bool ex_dir(uint2 e) { return (e.x >> 31) != 0; }
 __device__ __forceinline__ int ex_i(uint2 e) { return (int)(e.x & 0x7fffffffu); }
 __device__ __forceinline__ int ex_j(uint2 e) { return (int)e.y; }
 
+// signed r-weighted face area from global memory (face inside the domain)
+__device__ __forceinline__ float face_area(const DevInst& I, bool dirR, int fi, int fj) {
+  return dirR ? gld(I.ar, (unsigned)(fj * I.Nx + fi)) : gld(I.ax, (unsigned)(fj * (I.Nx + 1) + fi));
+}
+
 // ------------------------------------------------------------------------------
 template <typename R, int TX, int TY>
 struct StageInvSmem {
@@ -178,6 +186,8 @@ struct StageSmem {
   static constexpr int NFX = (TX + 1) * TY, NFR = TX * (TY + 1);
   static constexpr int TW = TX + 2, THh = TY + 2;  // T1/T2 tile with 1-cell ring
   R P[8][PH][PW];
+  float AX[TY][TX + 1];                            // signed areas of the tile's x faces
+  float AR[TY + 1][TX];                            // ... and r faces (0 outside the domain)
   StageFluxSmem<R, TX, TY, VIS> u;
   static constexpr int MAXE = MAX_EXTRA_FACES;     // faces of merged cells outside the tile
   static constexpr int MAXF = MAX_FAR_CELLS;       // cells those faces reach beyond the halo
@@ -187,9 +197,8 @@ struct StageSmem {
   R PF[8][MAXF];
   unsigned char fmap[WH][WW];                      // far-cell slot by window position
   int nEx, nF;
-  double red[16];
-  double redd[16];
-  int last;
+  double red[TX * TY / 64];                        // one slot per wave of the block reductions
+  double redd[TX * TY / 64];
 };
 
 // W: minimum waves per SIMD the compiler must fit for stage 1 and the stage 2 kernel
@@ -229,11 +238,58 @@ __global__ void __launch_bounds__(TX* TY, (STAGE == 2 && PHYS) ? 1 : W) k_stage(
   double rloc = 0.0;
   if (active) {
     // ---- 1. primitive tile with ghost halo (FS_PRIM); this tile's extra faces
-    for (int idx = tid; idx < S::PW * S::PH; idx += NT) {
-      const int li = idx % S::PW, lj = idx / S::PW;
-      const Prim<R> q = prim_at<R>(I, Uin, i0 - 2 + li, j0 - 2 + lj, tsel);
-      sm.P[0][lj][li] = q.r; sm.P[1][lj][li] = q.u; sm.P[2][lj][li] = q.v; sm.P[3][lj][li] = q.p;
-      sm.P[4][lj][li] = q.Z; sm.P[5][lj][li] = q.F; sm.P[6][lj][li] = q.G; sm.P[7][lj][li] = q.O;
+    // this thread's own cell: its code word, group geometry and state, for the update
+    // phase; loaded here so that their latency overlaps the tile load and the face loop
+    const int tx = tid % TX, ty = tid / TX;
+    const int ci = i0 + tx, cj = j0 + ty;
+    const bool inDom = ci < Nx && cj < Nr;
+    const long long k = (long long)cj * Nx + ci;
+    uint32_t code = C_IS_SLAVE;
+    float4 gg = make_float4(0.f, 0.f, 0.f, 0.f);
+    R a[4], b[4];
+    if (inDom) {
+      code = gld(I.code, (unsigned)k);
+      gg = gld(I.gg, (unsigned)k);
+      for (int q = 0; q < 4; ++q) { a[q] = gld(Uin + q * Ns, (unsigned)k); b[q] = gld(Uin + (4 + q) * Ns, (unsigned)k); }
+    }
+    // every load of this thread's cells and face areas is issued before any is used
+    constexpr int CPT = (S::PW * S::PH + NT - 1) / NT, FPT = (S::NFX + S::NFR + NT - 1) / NT;
+    CellRaw<R> raw[CPT];
+    float area[FPT];
+#pragma unroll
+    for (int r = 0; r < CPT; ++r) {
+      const int idx = tid + r * NT;
+      if (idx < S::PW * S::PH) raw[r] = cell_src_load<R>(I, Uin, i0 - 2 + idx % S::PW, j0 - 2 + idx / S::PW);
+    }
+#pragma unroll
+    for (int r = 0; r < FPT; ++r) {
+      const int f = tid + r * NT;
+      area[r] = 0.f;
+      if (f < S::NFX + S::NFR) {
+        const bool dirR = f >= S::NFX;
+        const int g = dirR ? f - S::NFX : f;
+        const int fi = i0 + (dirR ? g % TX : g % (TX + 1)), fj = j0 + (dirR ? g / TX : g / (TX + 1));
+        if (dirR ? (fi < Nx && fj <= Nr) : (fi <= Nx && fj < Nr)) area[r] = face_area(I, dirR, fi, fj);
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < CPT; ++r) {
+      const int idx = tid + r * NT;
+      if (idx < S::PW * S::PH) {
+        const int li = idx % S::PW, lj = idx / S::PW;
+        const Prim<R> q = prim_from<R>(I, raw[r], i0 - 2 + li, j0 - 2 + lj, tsel);
+        sm.P[0][lj][li] = q.r; sm.P[1][lj][li] = q.u; sm.P[2][lj][li] = q.v; sm.P[3][lj][li] = q.p;
+        sm.P[4][lj][li] = q.Z; sm.P[5][lj][li] = q.F; sm.P[6][lj][li] = q.G; sm.P[7][lj][li] = q.O;
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < FPT; ++r) {
+      const int f = tid + r * NT;
+      if (f < S::NFX + S::NFR) {
+        const bool dirR = f >= S::NFX;
+        const int g = dirR ? f - S::NFX : f;
+        if (dirR) sm.AR[g / TX][g % TX] = area[r]; else sm.AX[g / (TX + 1)][g % (TX + 1)] = area[r];
+      }
     }
     // extra faces of merged cells outside the tile, and the cells they reach beyond
     // the halo (primitives loaded here in parallel, so no thread waits on them later)
@@ -267,6 +323,8 @@ __global__ void __launch_bounds__(TX* TY, (STAGE == 2 && PHYS) ? 1 : W) k_stage(
     };
     // primitives by global cell index: the LDS tile, which always covers in-tile faces
     auto tileP = [&](int gi, int gj) { return P_at(gi - (i0 - 2), gj - (j0 - 2)); };
+    auto tileA = [&](bool dirR, int fi, int fj) { return dirR ? sm.AR[fj - j0][fi - i0] : sm.AX[fj - j0][fi - i0]; };
+    auto globA = [&](bool dirR, int fi, int fj) { return face_area(I, dirR, fi, fj); };
     // ... and, for faces of merged cells outside the tile, the preloaded far cells
     // (slot from the window map), else global memory
     auto anyP = [&](int gi, int gj) {
@@ -294,7 +352,7 @@ __global__ void __launch_bounds__(TX* TY, (STAGE == 2 && PHYS) ? 1 : W) k_stage(
         const int g = dirR ? f - S::NFX : f;
         const int fx = dirR ? g % TX : g % (TX + 1);
         const int fy = dirR ? g / TX : g / (TX + 1);
-        face_flux_w<R>(I, dirR, i0 + fx, j0 + fy, tileP, F, FY, FO);
+        face_flux_w<R>(I, dirR, i0 + fx, j0 + fy, tileP, tileA, F, FY, FO);
         if (dirR) {
           for (int q = 0; q < 4; ++q) { sm.u.inv.Fr[q][fy][fx] = F[q]; sm.u.inv.Fr[4 + q][fy][fx] = FY[q]; }
           sm.u.inv.Fr[8][fy][fx] = FO;
@@ -305,7 +363,7 @@ __global__ void __launch_bounds__(TX* TY, (STAGE == 2 && PHYS) ? 1 : W) k_stage(
       } else {
         const int e = f - S::NFX - S::NFR;
         const uint2 ef = sm.ex[e];
-        face_flux_w<R>(I, ex_dir(ef), ex_i(ef), ex_j(ef), anyP, F, FY, FO);
+        face_flux_w<R>(I, ex_dir(ef), ex_i(ef), ex_j(ef), anyP, globA, F, FY, FO);
         for (int q = 0; q < 4; ++q) { gst(exFlux, e * 16 + q, F[q]); gst(exFlux, e * 16 + 4 + q, FY[q]); }
         gst(exFlux, e * 16 + 8, FO);
       }
@@ -327,16 +385,11 @@ __global__ void __launch_bounds__(TX* TY, (STAGE == 2 && PHYS) ? 1 : W) k_stage(
         for (int q = 0; q < 9; ++q) F9[q] = gld(exFlux, slot * 16 + q);
         ++slot;
       } else {
-        face_flux_w<R>(I, dirR, fi, fj, anyP, F9, F9 + 4, F9[8]);
+        face_flux_w<R>(I, dirR, fi, fj, anyP, globA, F9, F9 + 4, F9[8]);
       }
     };
 
     // ---- 3. merged-group finite-volume update (FS_UPDATE)
-    const int tx = tid % TX, ty = tid / TX;
-    const int ci = i0 + tx, cj = j0 + ty;
-    const bool inDom = ci < Nx && cj < Nr;
-    const long long k = (long long)cj * Nx + ci;
-    const uint32_t code = inDom ? gld(I.code, (unsigned)k) : C_IS_SLAVE;
     const bool solid = (code & C_SOLID) != 0;
     const bool host = inDom && !solid && !(code & C_IS_SLAVE);
     R A[4], B[4];         // group state after the stage
@@ -345,7 +398,6 @@ __global__ void __launch_bounds__(TX* TY, (STAGE == 2 && PHYS) ? 1 : W) k_stage(
     int mslot[5];         // first extra-face slot of each member (-1: faces in the tile or none)
     bool guard = false;   // NaN guard hit
     R rho_old = R(0);
-    float4 gg = make_float4(0.f, 0.f, 0.f, 0.f);
     Prim<R> qS;           // state after the stage, as primitives (physics input)
     if (host) {
       mi[0] = ci; mj[0] = cj; mslot[0] = -1; nm = 1;
@@ -381,9 +433,6 @@ __global__ void __launch_bounds__(TX* TY, (STAGE == 2 && PHYS) ? 1 : W) k_stage(
         }
         LO += -(Fe_[8] - Fw[8]) - (Fn[8] - Fs[8]);
       }
-      gg = gld(I.gg, (unsigned)k);
-      R a[4], b[4];
-      for (int q = 0; q < 4; ++q) { a[q] = gld(Uin + q * Ns, (unsigned)k); b[q] = gld(Uin + (4 + q) * Ns, (unsigned)k); }
       const Prim<R> q = to_prim<R>(a[0], a[1], a[2], a[3], b[0], b[1], b[2], b[3]);
       const R Wx = R(gg.z), Wr = R(gg.w);
       const R wl = sqrt(Wx * Wx + Wr * Wr);
@@ -478,7 +527,7 @@ __global__ void __launch_bounds__(TX* TY, (STAGE == 2 && PHYS) ? 1 : W) k_stage(
             const int g = dirR ? f - S::NFX : f;
             const int fx = dirR ? g % TX : g % (TX + 1);
             const int fy = dirR ? g / TX : g / (TX + 1);
-            vface_w<R>(I, dirR, i0 + fx, j0 + fy, tileP, tileT, F, FY);
+            vface_w<R>(I, dirR, i0 + fx, j0 + fy, tileP, tileT, tileA, F, FY);
             if (dirR) {
               for (int q = 0; q < 4; ++q) sm.u.vis.Vr[q][fy][fx] = F[q];
               for (int q = 0; q < 3; ++q) sm.u.vis.Vr[4 + q][fy][fx] = FY[q];
@@ -489,7 +538,7 @@ __global__ void __launch_bounds__(TX* TY, (STAGE == 2 && PHYS) ? 1 : W) k_stage(
           } else {
             const int e = f - S::NFX - S::NFR;
             const uint2 ef = sm.ex[e];
-            vface_w<R>(I, ex_dir(ef), ex_i(ef), ex_j(ef), anyP, anyT, F, FY);
+            vface_w<R>(I, ex_dir(ef), ex_i(ef), ex_j(ef), anyP, anyT, globA, F, FY);
             for (int q = 0; q < 4; ++q) gst(exFlux, e * 16 + 9 + q, F[q]);
             for (int q = 0; q < 3; ++q) gst(exFlux, e * 16 + 13 + q, FY[q]);
           }
@@ -508,7 +557,7 @@ __global__ void __launch_bounds__(TX* TY, (STAGE == 2 && PHYS) ? 1 : W) k_stage(
           for (int q = 0; q < 7; ++q) F7[q] = gld(exFlux, slot * 16 + 9 + q);
           ++slot;
         } else {
-          vface_w<R>(I, dirR, fi, fj, anyP, anyT, F7, F7 + 4);
+          vface_w<R>(I, dirR, fi, fj, anyP, anyT, globA, F7, F7 + 4);
         }
       };
       if (host) {
