@@ -171,6 +171,25 @@ int nz_spray_set_count(nz_solver* s, int nparcels);
 /* parcel read-back for tests: PA, PB each nparcels*4 floats (spray.js:4-7) */
 int nz_spray_read(nz_solver* s, float* PA, float* PB);
 
+/* Derived display fields, FS_DERIVED (render.js:60-134) on the device: per cell
+ * (t, exhaust fraction Z, normalised temperature, Mach) of the current state, with
+ * solid cells (0, 0, 0, -1). The view parameters are those render.js derive() passes
+ * (render.js:420-435, from app.js:379-382). field: 0 T, 1 Mach, 2 schlieren,
+ * 3 log10 p, 4 log10 rho, 5 speed, 6 vorticity, 7 Z, 8 (as 0), 9 log10 mu_eff/mu,
+ * 10 burnt fraction, 11 F, 12 O, 13 burn rate. Fields 9 and 13 use vis.T2, which
+ * the library recomputes from the last stage-1 state (as NZ_FIELD_T2). */
+typedef struct nz_derived_params {
+  int32_t field;
+  float range[2];          /* display range (t = (v - range[0]) / (range[1] - range[0])) */
+  float RaN;               /* normalised air gas constant (P.gas[0]) */
+  float Rs[3];             /* P.Rs */
+  float chemK[4];          /* P.chemK */
+  float Tc, Ta;            /* chamber and ambient temperature, K */
+  float schlGain;          /* schlieren sensitivity */
+  float dx;                /* cell size, m */
+} nz_derived_params;
+int nz_read_derived(nz_solver* s, const nz_derived_params* d, float* out);
+
 /* Integrated quantities, summed on the device in FP64.
  * exitPerformance (simsetup.js:314-337), chamberPerformance (239-250),
  * cellPrimSI at the chamber probe (338-348, app.js:537). */

@@ -17,6 +17,7 @@
 //   FS_VFLUX_X / FS_VFLUX_R     solver.js:490-539  -> vflux_pass
 //   FS_SOURCE                   solver.js:540-663  -> source_pass
 //   FS_WALLROW                  solver.js:665-674  -> wall_row
+//   FS_DERIVED                  render.js:60-134   -> derived
 //   GPUSolver.step schedule     solver.js:849-891  -> step
 //   FS_SPRAY_UPDATE             spray.js:19-122    -> spray_update
 //   VS/FS_SPRAY_DEP             spray.js:125-149   -> spray_deposit (serial, in parcel order)
@@ -788,6 +789,75 @@ struct Oracle {
     }
   }
 
+  // ---------------- FS_DERIVED (render.js:60-134), the display fields of render.js derive()
+  void derived(const nz_derived_params& d, double velScale, float* out) const {
+    const std::vector<T>& A = SA[cur];
+    const std::vector<T>& B = SB[cur];
+    auto idx = [&](int a, int b) { return (size_t)std::min(std::max(b, 0), Nr - 1) * Nx + std::min(std::max(a, 0), Nx - 1); };
+    auto solid = [&](int a, int b) { return mask[idx(a, b)] != 0; };
+    auto P = [&](int a, int b) { const size_t k = idx(a, b); return toPrim(load4(A, k), load4(B, k)); };
+    auto lrho = [&](int a, int b) { return std::log(std::max(A[idx(a, b < 0 ? -b - 1 : b) * 4], T(1e-7))); };
+    const int f = d.field == 8 ? 0 : d.field;
+    const T vel = Tp<T>((float)velScale);
+    for (int j = 0; j < Nr; ++j)
+      for (int i = 0; i < Nx; ++i) {
+        float* o = out + ((size_t)j * Nx + i) * 4;
+        if (solid(i, j)) { o[0] = 0.f; o[1] = 0.f; o[2] = 0.f; o[3] = -1.f; continue; }
+        const Prim<T> q = P(i, j);
+        const T rho = q.w.x, p = q.w.w, Z = q.s.x, F = q.s.y, O = q.s.w;
+        const T c = soundSpeed(q);
+        const T spd = std::sqrt(q.w.y * q.w.y + q.w.z * q.w.z);
+        const T M = spd / c;
+        const T Rm = (T(1) - Z) * Tp<T>(d.RaN) + F * Tp<T>(d.Rs[0]) + O * Tp<T>(d.Rs[1]) +
+                     std::max(Z - F - O, T(0)) * Tp<T>(d.Rs[2]);
+        const T TT = p / (rho * Rm) * Tp<T>(288.15f);
+        T v = TT;
+        switch (f) {
+          case 1: v = M; break;
+          case 2: {
+            const T l0 = lrho(i, j);
+            const T le = solid(i + 1, j) ? l0 : lrho(i + 1, j), lw = solid(i - 1, j) ? l0 : lrho(i - 1, j);
+            const T ln = solid(i, j + 1) ? l0 : lrho(i, j + 1), ls = (j > 0 && solid(i, j - 1)) ? l0 : lrho(i, j - 1);
+            v = T(1) - std::exp(-Tp<T>(d.schlGain) * std::sqrt((le - lw) * (le - lw) + (ln - ls) * (ln - ls)) * T(0.5));
+            break;
+          }
+          case 3: v = std::log(p * T(101325)) / std::log(T(10)); break;
+          case 4: v = std::log(rho * T(1.225f)) / std::log(T(10)); break;
+          case 5: v = spd * vel; break;
+          case 6: {
+            const Prim<T> e = P(i + 1, j), w = P(i - 1, j), n = P(i, j + 1);
+            const T su = j > 0 ? P(i, j - 1).w.y : q.w.y;
+            v = std::fabs(T(0.5) * (e.w.z - w.w.z) - T(0.5) * (n.w.y - su)) * vel / Tp<T>(d.dx);
+            break;
+          }
+          case 7: v = Z; break;
+          case 9: {
+            const T m0 = T2[((size_t)j * Nx + i) * 4], m3 = T2[((size_t)j * Nx + i) * 4 + 3];
+            v = m3 > T(0) ? std::log(std::max(m0 / m3, T(1))) / std::log(T(10)) : T(0);
+            break;
+          }
+          case 10: v = Z > T(0.003f) ? (Z - F) / Z : T(0); break;
+          case 11: v = F; break;
+          case 12: v = O; break;
+          case 13: {
+            const T m0 = T2[((size_t)j * Nx + i) * 4], m2 = T2[((size_t)j * Nx + i) * 4 + 2];
+            const T avail = std::min(F, (O + T(0.233f) * std::max(T(1) - Z, T(0))) / Tp<T>(d.chemK[1]));
+            const T kin = Tp<T>(d.chemK[2]) * p * std::exp(T(-9000) / std::max(TT, T(200)));
+            v = std::log(T(1) + T(1e5f) * avail * std::min(std::max(m2, T(4) * m0 / rho), kin)) / std::log(T(10));
+            break;
+          }
+          default: break;
+        }
+        T t = (v - Tp<T>(d.range[0])) / (Tp<T>(d.range[1]) - Tp<T>(d.range[0]));
+        if (f == 2) t = v;
+        if (f == 6) t = std::log(T(1) + T(20) * std::max(v, T(0)) / Tp<T>(d.range[1])) / std::log(T(21));
+        o[0] = (float)t;
+        o[1] = (float)Z;
+        o[2] = (float)((TT - Tp<T>(d.Ta)) / std::max(Tp<T>(d.Tc) - Tp<T>(d.Ta), T(1)));
+        o[3] = (float)M;
+      }
+  }
+
   // ---------------- spray (spray.js:19-179, 256-297)
   static uint32_t pcg(uint32_t v) {
     const uint32_t s = v * 747796405u + 2891336453u;
@@ -1108,6 +1178,11 @@ void oracle_read_parcels(void* p, float* PA, float* PB) {
   Handle* h = static_cast<Handle*>(p);
   if (h->prec == 64) { copy_out(PA, h->d.PA[h->d.spCur], h->d.PA[0].size()); copy_out(PB, h->d.PB[h->d.spCur], h->d.PB[0].size()); }
   else { copy_out(PA, h->f.PA[h->f.spCur], h->f.PA[0].size()); copy_out(PB, h->f.PB[h->f.spCur], h->f.PB[0].size()); }
+}
+// FS_DERIVED display fields of the current state (velScale: U_REF)
+void oracle_derived(void* p, const nz_derived_params* d, double velScale, float* out) {
+  Handle* h = static_cast<Handle*>(p);
+  if (h->prec == 64) h->d.derived(*d, velScale, out); else h->f.derived(*d, velScale, out);
 }
 // place parcel k (a = x, r, u, v; b = d, m, type, x_impinge), for single-droplet tests
 void oracle_set_parcel(void* p, int k, const float* a, const float* b) {

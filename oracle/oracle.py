@@ -50,6 +50,7 @@ def lib():
             "oracle_wall_row": (None, [vp, fp]),
             "oracle_read_parcels": (None, [vp, fp, fp]),
             "oracle_set_parcel": (None, [vp, C.c_int, fp, fp]),
+            "oracle_derived": (None, [vp, C.c_void_p, C.c_double, fp]),
             "oracle_pcg": (C.c_int, [C.c_uint32]),
         }
         for n, (r, a) in sig.items():
@@ -133,6 +134,12 @@ class Oracle:
         a = np.ascontiguousarray(a, np.float32)
         b = np.ascontiguousarray(b, np.float32)
         lib().oracle_set_parcel(self._h, k, _f(a), _f(b))
+
+    def derived(self, d, vel_scale):
+        """FS_DERIVED fields of the current state; d is an nz_derived_params structure."""
+        out = np.zeros(self.Nx * self.Nr * 4, np.float32)
+        lib().oracle_derived(self._h, C.addressof(d), vel_scale, _f(out))
+        return out
 
     def parcels(self, n):
         PA = np.zeros(n * 4, np.float32)

@@ -309,6 +309,72 @@ __global__ void k_spray_reset(float* __restrict__ parc, int NP, float* __restric
 }
 
 // FP64 integrals of the current state: exit plane, throat plane, chamber probe, totals
+// FS_DERIVED (render.js:60-134): one display field per cell plus (Z, Tn, M); neighbours
+// clamped into the grid, the axis mirrored for the density gradient and the vorticity.
+// Constants are the shader's float literals and uniforms, also in the FP64 build.
+template <typename R>
+__global__ void __launch_bounds__(256) k_derived(DevInst I, DerivedC d, const float* __restrict__ T2, float* __restrict__ out) {
+  const int Nx = I.Nx, Nr = I.Nr;
+  const long long N = (long long)Nx * Nr;
+  const long long k = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (k >= N) return;
+  const int i = (int)(k % Nx), j = (int)(k / Nx);
+  const R* __restrict__ U = static_cast<const R*>(I.U0);
+  const long long Ns = I.Ns;
+  auto cell = [&](int a, int b) { return (long long)min(max(b, 0), Nr - 1) * Nx + min(max(a, 0), Nx - 1); };
+  auto solid = [&](int a, int b) { return (gld(I.code, (unsigned)cell(a, b)) & C_SOLID) != 0; };
+  auto P = [&](int a, int b) { return load_prim<R>(U, Ns, cell(a, b)); };
+  auto lrho = [&](int a, int b) { return log(fmax(gld(U, (unsigned)cell(a, b < 0 ? -b - 1 : b)), R(1e-7))); };
+  float* o = out + k * 4;
+  if (solid(i, j)) { o[0] = 0.f; o[1] = 0.f; o[2] = 0.f; o[3] = -1.f; return; }
+  const Prim<R> q = P(i, j);
+  const R rho = q.r, p = q.p;
+  const R c = sound_speed(q);
+  const R spd = sqrt(q.u * q.u + q.v * q.v);
+  const R M = spd / c;
+  const R Rm = (R(1) - q.Z) * R(d.RaN) + q.F * R(d.Rs[0]) + q.O * R(d.Rs[1]) + fmax(q.Z - q.F - q.O, R(0)) * R(d.Rs[2]);
+  const R T = p / (rho * Rm) * R(288.15f);
+  const int f = d.field == 8 ? 0 : d.field;
+  R v = T;
+  if (f == 1) v = M;
+  else if (f == 2) {
+    const R l0 = lrho(i, j);
+    const R le = solid(i + 1, j) ? l0 : lrho(i + 1, j);
+    const R lw = solid(i - 1, j) ? l0 : lrho(i - 1, j);
+    const R ln = solid(i, j + 1) ? l0 : lrho(i, j + 1);
+    const R ls = (j > 0 && solid(i, j - 1)) ? l0 : lrho(i, j - 1);
+    const R g = sqrt((le - lw) * (le - lw) + (ln - ls) * (ln - ls)) * R(0.5);
+    v = R(1) - exp(-R(d.schl) * g);
+  } else if (f == 3) v = log(p * R(101325)) / log(R(10));
+  else if (f == 4) v = log(rho * R(1.225f)) / log(R(10));
+  else if (f == 5) v = spd * R(d.vel);
+  else if (f == 6) {
+    const Prim<R> e = P(i + 1, j), w = P(i - 1, j), n = P(i, j + 1);
+    const R su = j > 0 ? P(i, j - 1).u : q.u;
+    const R om = R(0.5) * (e.v - w.v) - R(0.5) * (n.u - su);
+    v = fabs(om) * R(d.vel) / R(d.dx);
+  } else if (f == 7) v = q.Z;
+  else if (f == 9) {
+    const float m0 = T2[k * 4], m3 = T2[k * 4 + 3];
+    v = m3 > 0.f ? log(fmax(R(m0) / R(m3), R(1))) / log(R(10)) : R(0);
+  } else if (f == 10) v = q.Z > R(0.003f) ? (q.Z - q.F) / q.Z : R(0);
+  else if (f == 11) v = q.F;
+  else if (f == 12) v = q.O;
+  else if (f == 13) {
+    const R m0 = R(T2[k * 4]), m2 = R(T2[k * 4 + 2]);
+    const R avail = fmin(q.F, (q.O + R(0.233f) * fmax(R(1) - q.Z, R(0))) / R(d.chemK[1]));
+    const R kin = R(d.chemK[2]) * p * exp(R(-9000) / fmax(T, R(200)));
+    v = log(R(1) + R(1e5) * avail * fmin(fmax(m2, R(4) * m0 / rho), kin)) / log(R(10));
+  }
+  R t = (v - R(d.r0)) / (R(d.r1) - R(d.r0));
+  if (f == 2) t = v;
+  if (f == 6) t = log(R(1) + R(20) * fmax(v, R(0)) / R(d.r1)) / log(R(21));
+  o[0] = (float)t;
+  o[1] = (float)q.Z;
+  o[2] = (float)((T - R(d.Ta)) / fmax(R(d.Tc) - R(d.Ta), R(1)));
+  o[3] = (float)M;
+}
+
 template <typename R>
 __global__ void __launch_bounds__(256) k_integrals(DevInst I, int iExit, int jExit, int iThroat, int iChamber, double* __restrict__ out) {
   __shared__ double red[16];
@@ -491,6 +557,13 @@ hipError_t launch_pack_f64(bool fp64, const void* U, long long Ns, long long N, 
 hipError_t launch_spray_reset(float* parc, int NP, float* src, long long Ns, hipStream_t s) {
   const long long n = NP > Ns ? NP : Ns;
   hipLaunchKernelGGL(k_spray_reset, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, parc, NP, src, Ns);
+  return hipGetLastError();
+}
+hipError_t launch_derived(bool fp64, const DevInst& I, const DerivedC& d, const float* T2, float* out, hipStream_t s) {
+  const long long N = (long long)I.Nx * I.Nr;
+  const dim3 g((unsigned)((N + 255) / 256)), b(256);
+  if (fp64) hipLaunchKernelGGL((k_derived<double>), g, b, 0, s, I, d, T2, out);
+  else hipLaunchKernelGGL((k_derived<float>), g, b, 0, s, I, d, T2, out);
   return hipGetLastError();
 }
 hipError_t launch_integrals(bool fp64, const DevInst& I, int iExit, int jExit, int iThroat, int iChamber, double* out, hipStream_t s) {

@@ -125,6 +125,12 @@ __device__ __forceinline__ void gst(T* base, unsigned i, T v) {
   *(NZ_GLOBAL T*)((NZ_GLOBAL char*)base + off) = v;
 }
 
+// view parameters of the FS_DERIVED display fields (render.js:420-435)
+struct DerivedC {
+  int field;
+  float r0, r1, RaN, Rs[3], chemK[4], Tc, Ta, schl, dx, vel;  // vel: U_REF as render.js passes it
+};
+
 template <typename R>
 struct Prim {
   R r, u, v, p, Z, F, G, O;

@@ -26,6 +26,7 @@ hipError_t launch_unpack(bool fp64, void* U, long long Ns, long long N, const fl
 hipError_t launch_step_info(const InstState* st, float* out, hipStream_t s);
 hipError_t launch_pack_f64(bool fp64, const void* U, long long Ns, long long N, double* A, double* B, hipStream_t s);
 hipError_t launch_spray_reset(float* parc, int NP, float* src, long long Ns, hipStream_t s);
+hipError_t launch_derived(bool fp64, const DevInst& I, const DerivedC& d, const float* T2, float* out, hipStream_t s);
 hipError_t launch_integrals(bool fp64, const DevInst& I, int iExit, int jExit, int iThroat, int iChamber, double* out, hipStream_t s);
 
 }  // namespace nzk

@@ -252,6 +252,17 @@ __global__ void __launch_bounds__(TX* TY, (STAGE == 2 && PHYS) ? 1 : W) k_stage(
       gg = gld(I.gg, (unsigned)k);
       for (int q = 0; q < 4; ++q) { a[q] = gld(Uin + q * Ns, (unsigned)k); b[q] = gld(Uin + (4 + q) * Ns, (unsigned)k); }
     }
+    // eddy-viscosity length scales of this thread's cells of the viscous ring tile
+    constexpr int LPT = (S::TW * S::THh + NT - 1) / NT;
+    float ltv[LPT];
+    if constexpr (STAGE == 2 && PHYS) {
+#pragma unroll
+      for (int r = 0; r < LPT; ++r) {
+        const int idx = tid + r * NT;
+        const int gi = i0 - 1 + idx % S::TW, gj = j0 - 1 + idx / S::TW;
+        ltv[r] = (idx < S::TW * S::THh && gi >= 0 && gi < Nx && gj >= 0 && gj < Nr) ? gld(I.lt, (unsigned)(gj * Nx + gi)) : 0.f;
+      }
+    }
     // every load of this thread's cells and face areas is issued before any is used
     constexpr int CPT = (S::PW * S::PH + NT - 1) / NT, FPT = (S::NFX + S::NFR + NT - 1) / NT;
     CellRaw<R> raw[CPT];
@@ -494,14 +505,17 @@ __global__ void __launch_bounds__(TX* TY, (STAGE == 2 && PHYS) ? 1 : W) k_stage(
     if constexpr (STAGE == 2 && PHYS) {
     if (c.visc || c.chem) {  // per instance: a batch mixes both kinds
       __syncthreads();  // inviscid fluxes consumed; reuse their LDS
-      for (int idx = tid; idx < S::TW * S::THh; idx += NT) {
+#pragma unroll
+      for (int r = 0; r < LPT; ++r) {
+        const int idx = tid + r * NT;
+        if (idx >= S::TW * S::THh) break;
         const int ri = idx % S::TW, rj = idx / S::TW;
         const int li = ri + 1, lj = rj + 1;
         const int gi = i0 - 2 + li, gj = j0 - 2 + lj;
         R T1[4] = {0, 0, 0, 0}, T2[4] = {0, 0, 0, 0};
         if (gi >= 0 && gi < Nx && gj >= 0 && gj < Nr && sm.P[0][lj][li] >= R(0)) {
           vcell<R>(P_at(li, lj), P_at(li + 1, lj), P_at(li - 1, lj), P_at(li, lj + 1), P_at(li, lj - 1), gi, gj,
-                   R(gld(I.lt, (unsigned)(gj * Nx + gi))), dt, c, T1, T2);
+                   R(ltv[r]), dt, c, T1, T2);
         }
         for (int q = 0; q < 4; ++q) { sm.u.vis.T[q][rj][ri] = T1[q]; sm.u.vis.T[4 + q][rj][ri] = T2[q]; }
       }

@@ -583,6 +583,30 @@ int nz_set_probes(nz_solver* s, const nz_probes* pr) {
   return 0;
 }
 
+int nz_read_derived(nz_solver* s, const nz_derived_params* d, float* out) {
+  if (int rc = check_solver(s, "nz_read_derived")) return rc;
+  if (!d || !out) return nz_fail(-1, "nz_read_derived: null argument");
+  if (d->field < 0 || d->field > 13) return nz_fail(-1, "nz_read_derived: field must be in [0, 13]");
+  const size_t N = (size_t)s->I.Nx * s->I.Nr;
+  if (int rc = ensure_staging(s, N * 8)) return rc;
+  nzk::DerivedC c;
+  c.field = d->field;
+  c.r0 = d->range[0]; c.r1 = d->range[1];
+  c.RaN = d->RaN;
+  for (int q = 0; q < 3; ++q) c.Rs[q] = d->Rs[q];
+  for (int q = 0; q < 4; ++q) c.chemK[q] = d->chemK[q];
+  c.Tc = d->Tc; c.Ta = d->Ta; c.schl = d->schlGain; c.dx = d->dx;
+  c.vel = (float)nz::U_REF;
+  const bool fp64 = s->prec == 64;
+  float* t2 = s->dstage + N * 4;
+  if (d->field == 9 || d->field == 13) HIPCHK(nzk::launch_t2(fp64, s->h, t2, s->stream), "derived: T2");
+  HIPCHK(nzk::launch_derived(fp64, s->h, c, t2, s->dstage, s->stream), "derived");
+  HIPCHK(hipMemcpyAsync(s->hpin, s->dstage, N * 16, hipMemcpyDeviceToHost, s->stream), "derived copy");
+  HIPCHK(hipStreamSynchronize(s->stream), "derived sync");
+  std::memcpy(out, s->hpin, N * 16);
+  return 0;
+}
+
 int nz_integrals_read(nz_solver* s, const nz_params* P, nz_integrals* out) {
   if (int rc = check_solver(s, "nz_integrals_read")) return rc;
   if (!out || !P) return nz_fail(-1, "nz_integrals_read: null argument");

@@ -93,6 +93,12 @@ class NzIntegrals(C.Structure):
                 ("F", "mdot", "Me", "pe", "Isp", "mdotThroat", "Pc", "Tc", "mass", "energy", "residual")]
 
 
+class NzDerivedParams(C.Structure):
+    _fields_ = [("field", C.c_int32), ("range", C.c_float * 2), ("RaN", C.c_float), ("Rs", C.c_float * 3),
+                ("chemK", C.c_float * 4), ("Tc", C.c_float), ("Ta", C.c_float), ("schlGain", C.c_float),
+                ("dx", C.c_float)]
+
+
 class NzBatchJob(C.Structure):
     _fields_ = [("Nx", C.c_int32), ("Nr", C.c_int32), ("mask", C.POINTER(C.c_uint8)),
                 ("geoA", C.POINTER(C.c_float)), ("geoG", C.POINTER(C.c_float)), ("geoM", C.POINTER(C.c_float)),
@@ -160,6 +166,7 @@ SIGNATURES = {
     "nz_spray_read": (C.c_int, [vp, fp, fp]),
     "nz_set_probes": (C.c_int, [vp, P(NzProbes)]),
     "nz_integrals_read": (C.c_int, [vp, P(NzParams), P(NzIntegrals)]),
+    "nz_read_derived": (C.c_int, [vp, P(NzDerivedParams), fp]),
     "nz_batch_create": (C.c_int, [P(vp), C.c_int, C.c_uint64, C.c_int]),
     "nz_batch_destroy": (C.c_int, [vp]),
     "nz_batch_instance_bytes": (C.c_uint64, [C.c_int, C.c_int, C.c_int, C.c_int]),

@@ -61,6 +61,19 @@ class GPUSpray:
         return PA, PB
 
 
+def derived_params(field, rng, P, Tc, Ta, schlGain, dx):
+    """nz_derived_params as app.js fills the view (app.js:379-382): gas constants and
+    kinetics from the parameter block P, chamber / ambient temperature in K."""
+    d = abi.NzDerivedParams()
+    d.field = int(field)
+    d.range[:] = [float(rng[0]), float(rng[1])]
+    d.RaN = P.gas[0]
+    d.Rs[:] = list(P.Rs)
+    d.chemK[:] = list(P.chemK)
+    d.Tc, d.Ta, d.schlGain, d.dx = float(Tc), float(Ta), float(schlGain), float(dx)
+    return d
+
+
 def tile_shapes(precision=32, waves=False):
     """[(TX, TY)] (or [(TX, TY, W)] with waves=True) of the stage kernels' tile shapes at
     this precision; index = shape id."""
@@ -241,6 +254,13 @@ class GPUSolver:
     def setProbes(self, iExit, jExit, iThroat, iChamber, dx):
         pr = abi.NzProbes(iExit, jExit, iThroat, iChamber, dx)
         check(lib().nz_set_probes(self._h, C.byref(pr)), "setProbes")
+
+    def derived(self, field, rng, P, Tc, Ta, schlGain, dx):
+        """render.js derive() (FS_DERIVED, render.js:60-134): per cell (t, Z, Tn, M)."""
+        d = derived_params(field, rng, P, Tc, Ta, schlGain, dx)
+        out = np.zeros(self.Nx * self.Nr * 4, np.float32)
+        check(lib().nz_read_derived(self._h, C.byref(d), _f(out)), "derived")
+        return out
 
     def integrals(self, P):
         out = abi.NzIntegrals()

@@ -296,3 +296,27 @@ def test_errors_are_raised():
     g.resize(64, 24)
     with pytest.raises(Exception):
         g._readRect("A", 60, 0, 10, 1)                   # outside the texture
+
+
+U_REF = (287.053 * 288.15) ** 0.5   # sqrt(P_REF / RHO_REF), physics.js:11-15
+DERIVED_FIELDS = {0: (250, 3600), 1: (0, 4), 2: (0, 1), 3: (3.5, 7), 4: (-2, 1.5), 5: (0, 3000), 6: (0, 2e5),
+                  7: (0, 1), 9: (0, 3), 10: (0, 1), 11: (0, 1), 12: (0, 1), 13: (0, 6)}
+
+
+@pytest.mark.parametrize("prec", [32, 64])
+def test_derived_display_fields(prec):
+    """FS_DERIVED on the device against the oracle's restatement, every field, after a
+    short hot-bell run with afterburning (FP32 rel-L2 <= 1e-4 per channel; FP64 1e-9)."""
+    from rnozzle.solver import derived_params
+    s = SimSetup(configs.hot_bell(Nx=192))
+    g, o = run_pair(s, 60, prec=prec)
+    tol = 1e-4 if prec == 32 else 1e-9
+    for f, rng in DERIVED_FIELDS.items():
+        out = g.derived(f, rng, s.P, 3500.0, s.info.atmT, 10.0, s.dx).reshape(-1, 4).astype(np.float64)
+        ref = o.derived(derived_params(f, rng, s.P, 3500.0, s.info.atmT, 10.0, s.dx), U_REF).reshape(-1, 4)
+        wall = ref[:, 3] < 0
+        assert np.array_equal(out[:, 3] < 0, wall)
+        for ch in range(4):
+            e = rel_l2(out[~wall, ch], ref[~wall, ch].astype(np.float64))
+            assert e <= tol, (f, ch, e)
+    say(f"[parity] derived fields FP{prec}: all {len(DERIVED_FIELDS)} fields within {tol}")

@@ -471,3 +471,54 @@ def test_afterburning_rates():
     Ta = np.log(rates[(0.25, 1.0)] / rates[(0.3125, 1.0)]) / (1 / T2 - 1 / T1)
     assert Ta == pytest.approx(9000.0, rel=1e-5)
     assert rates[(0.25, 1.0)] == pytest.approx(1.0 * 1.0 * np.exp(-9000.0 / T1), rel=1e-6)
+
+
+# ---------------------------------------------------------------- display fields
+def test_derived_fields_closed_forms():
+    """FS_DERIVED on states whose display values are known in closed form: temperature
+    p / (rho R) x 288.15 K, Mach |u| / c, log10 of p and rho in SI, speed in m/s, the
+    schlieren of an exponential density ramp (|grad ln rho| = alpha per cell), the
+    vorticity of the shear u = a r (a U_REF / dx), the burnt fraction (Z - F) / Z, and
+    the wall marker on solid cells."""
+    from rnozzle.solver import derived_params
+    U_REF = (287.053 * 288.15) ** 0.5
+    Nx, Nr = 40, 24
+    mask, geoA, geoG, geoM = box_geometry(Nx, Nr)
+    x = (np.arange(Nx) + 0.5)[None, :] * np.ones((Nr, 1))
+    r = (np.arange(Nr) + 0.5)[:, None] * np.ones((1, Nx))
+    alpha, a, u0 = 0.01, 2.0 ** -7, 0.25
+    rho = np.exp(alpha * x)
+    p = np.full((Nr, Nx), 0.8)
+    A, B = state_from_prims(rho, u0 + a * r, np.zeros((Nr, Nx)), p)
+    Bm = B.reshape(Nr, Nx, 4)
+    Bm[..., 0] = 0.4 * rho
+    Bm[..., 1] = 0.1 * rho
+    P = uniform_params()
+    o = orc.Oracle(Nx, Nr, 64)
+    o.set_geometry(mask, geoA, geoG, geoM)
+    o.set_state(A, Bm.ravel())
+    SA, SB = o.state_f64()
+    q = SA.reshape(Nr, Nx, 4)
+    rr, uu, pp = q[..., 0], q[..., 1] / q[..., 0], (q[..., 3] - 0.5 * q[..., 1] ** 2 / q[..., 0]) / 2.5
+    dx = 1e-3
+    rng = (0.0, 1.0)
+
+    def field(f):
+        return o.derived(derived_params(f, rng, P, 3000.0, 288.15, 10.0, dx), U_REF).reshape(Nr, Nx, 4)
+
+    inner = (slice(2, Nr - 3), slice(2, Nx - 3))
+    T = field(0)[..., 0]
+    np.testing.assert_allclose(T[inner], (pp / rr * 288.15)[inner], rtol=1e-6)
+    M = field(1)[..., 0]
+    np.testing.assert_allclose(M[inner], (uu / np.sqrt(1.4 * pp / rr))[inner], rtol=1e-6)
+    np.testing.assert_allclose(field(3)[..., 0][inner], np.log10(pp * 101325)[inner], rtol=1e-6)
+    np.testing.assert_allclose(field(4)[..., 0][inner], np.log10(rr * 1.225)[inner], rtol=1e-5, atol=1e-6)
+    np.testing.assert_allclose(field(5)[..., 0][inner], (uu * U_REF)[inner], rtol=1e-6)
+    schl = field(2)[..., 0]
+    np.testing.assert_allclose(schl[inner], 1 - np.exp(-10.0 * alpha), rtol=1e-4)
+    vort = field(6)[..., 0]       # t = log(1 + 20 v / range_hi) / log 21
+    np.testing.assert_allclose(vort[inner], np.log1p(20 * a * U_REF / dx / rng[1]) / np.log(21), rtol=1e-5)
+    np.testing.assert_allclose(field(10)[..., 0][inner], 0.75, rtol=1e-5)
+    out = field(0)
+    solid = mask.reshape(Nr, Nx) != 0
+    assert np.all(out[solid][:, 3] == -1) and np.all(out[~solid][:, 3] >= 0)
