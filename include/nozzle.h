@@ -351,6 +351,14 @@ int nz_setup_get_geometry(const nz_setup* s, uint8_t* mask, float* geoA, float* 
 int nz_setup_initial_state(const nz_setup* s, float* A, float* B);
 /* setup.injector.tab (simsetup.js:181-237); returns -1 if not injector mode */
 int nz_setup_get_injector(const nz_setup* s, float* tab);
+/* HUD read-outs on the device's near-wall row (nz_draw_wall_row + NZ_FIELD_WALLROW:
+ * Nx x (u, v, p, c), normalised). wallSeparation (simsetup.js:352-373): returns 1
+ * and fills *out when the bell has separated, 0 while attached. */
+typedef struct nz_separation { double x, frac, eps, pRatio, Msep; } nz_separation;
+int nz_wall_separation(const nz_setup* s, const float* row, double paSi, nz_separation* out);
+/* summerfieldPoint (simsetup.js:375-384): fraction of the bell where 1-D theory puts
+ * the wall pressure below 0.4 pa; returns 1 and fills *frac, or 0 if never */
+int nz_summerfield_point(const nz_setup* s, double PcSi, double gamma, double paSi, double* frac);
 /* contour radius at x (m) (buildContour.radiusAt, physics.js:190-197) */
 double nz_setup_radius_at(const nz_setup* s, double x);
 /* nozzleTheory (physics.js:75-106); out = {At,Ae,eps,Me,pe,Te,ue,cstar,mdot,F,Fvac,Isp,c0,uLim} */

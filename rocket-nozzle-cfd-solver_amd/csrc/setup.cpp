@@ -1093,6 +1093,56 @@ int nz_setup_get_injector(const nz_setup* sp, float* tab) {
   return 0;
 }
 
+int nz_wall_separation(const nz_setup* sp, const float* row, double paSi, nz_separation* out) {
+  if (!sp || !row || !out) return nz_fail(-1, "nz_wall_separation: null argument");
+  const Setup& S = *reinterpret_cast<const Setup*>(sp);
+  const Contour& C = S.contour;
+  const double dx = S.dx;
+  // walk upstream from the exit while the near-wall flow runs backwards or slows below
+  // 0.6 c; up to two attached cells are bridged (simsetup.js:352-360)
+  int i = S.iExit, gaps = 0, last = -1;
+  for (; i > S.iThroat + 2 && gaps <= 2; --i) {
+    const size_t k = (size_t)i * 4;
+    const double x = (i + 0.5) * dx;
+    const double slope = (C.radiusAt(x + 0.5 * dx) - C.radiusAt(x - 0.5 * dx)) / dx;
+    const double ut = (row[k] + slope * row[k + 1]) / std::hypot(1.0, slope);
+    if (ut < 0 || std::hypot((double)row[k], (double)row[k + 1]) < 0.6 * row[k + 3]) { last = i; gaps = 0; }
+    else if (last >= 0 || S.iExit - i > 2) gaps++;
+  }
+  if (last < 0 || S.iExit - last < 3) return 0;
+  // wall pressure ahead of the separation shock: the lowest in the 30 cells upstream
+  double pSep = INFINITY;
+  size_t kS = (size_t)last * 4;
+  for (int q = last; q >= std::max(S.iThroat + 1, last - 30); --q)
+    if (row[(size_t)q * 4 + 2] < pSep) { pSep = row[(size_t)q * 4 + 2]; kS = (size_t)q * 4; }
+  const double Msep = std::hypot((double)row[kS], (double)row[kS + 1]) / std::max((double)row[kS + 3], 1e-6);
+  if (Msep < 1.05) return 0;   // nothing supersonic ahead of it (the nozzle has not started)
+  const double x = (last + 0.5) * dx, R = C.radiusAt(x);
+  out->x = x;
+  out->frac = (x - C.xt) / (C.xe - C.xt);
+  out->eps = (R * R) / (C.Rt * C.Rt);
+  out->pRatio = pSep * P_REF / std::max(paSi, 1e-6);
+  out->Msep = Msep;
+  return 1;
+}
+
+int nz_summerfield_point(const nz_setup* sp, double PcSi, double gamma, double paSi, double* frac) {
+  if (!sp || !frac) return nz_fail(-1, "nz_summerfield_point: null argument");
+  const Contour& C = reinterpret_cast<const Setup*>(sp)->contour;
+  // first station of the divergent section where 1-D isentropic wall pressure drops
+  // below 0.4 of ambient (simsetup.js:375-384)
+  const int nS = 200;
+  for (int k = 0; k <= nS; ++k) {
+    const double x = C.xt + (C.xe - C.xt) * k / nS, R = C.radiusAt(x);
+    const double M = machFromAreaRatio(std::max((R * R) / (C.Rt * C.Rt), 1.0001), gamma, true);
+    if (PcSi * std::pow(1 + 0.5 * (gamma - 1) * M * M, -gamma / (gamma - 1)) < 0.4 * paSi) {
+      *frac = (double)k / nS;
+      return 1;
+    }
+  }
+  return 0;
+}
+
 double nz_setup_radius_at(const nz_setup* sp, double x) {
   if (!sp) return 0;
   return reinterpret_cast<const Setup*>(sp)->contour.radiusAt(x);

@@ -99,6 +99,10 @@ class NzDerivedParams(C.Structure):
                 ("dx", C.c_float)]
 
 
+class NzSeparation(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("x", "frac", "eps", "pRatio", "Msep")]
+
+
 class NzBatchJob(C.Structure):
     _fields_ = [("Nx", C.c_int32), ("Nr", C.c_int32), ("mask", C.POINTER(C.c_uint8)),
                 ("geoA", C.POINTER(C.c_float)), ("geoG", C.POINTER(C.c_float)), ("geoM", C.POINTER(C.c_float)),
@@ -192,6 +196,8 @@ SIGNATURES = {
     "nz_setup_initial_state": (C.c_int, [vp, fp, fp]),
     "nz_setup_get_injector": (C.c_int, [vp, fp]),
     "nz_setup_radius_at": (C.c_double, [vp, C.c_double]),
+    "nz_wall_separation": (C.c_int, [vp, fp, C.c_double, P(NzSeparation)]),
+    "nz_summerfield_point": (C.c_int, [vp, C.c_double, C.c_double, C.c_double, dp]),
     "nz_nozzle_theory": (C.c_int, [C.c_double] * 7 + [dp]),
     "nz_atmosphere": (None, [C.c_double, dp]),
 }

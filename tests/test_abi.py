@@ -64,7 +64,7 @@ def test_struct_layout_matches_header(tmp_path):
     mirrors = {"nz_params": abi.NzParams, "nz_read_req": abi.NzReadReq, "nz_spray_stats_t": abi.NzSprayStats,
                "nz_probes": abi.NzProbes, "nz_integrals": abi.NzIntegrals, "nz_config": abi.NzConfig,
                "nz_setup_info": abi.NzSetupInfo, "nz_batch_job": abi.NzBatchJob, "nz_batch_result": abi.NzBatchResult,
-               "nz_derived_params": abi.NzDerivedParams}
+               "nz_derived_params": abi.NzDerivedParams, "nz_separation": abi.NzSeparation}
     lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "nozzle.h"', "int main(void) {"]
     for cname, py in mirrors.items():
         lines.append(f'  printf("{cname} sizeof %zu\\n", sizeof({cname}));')
