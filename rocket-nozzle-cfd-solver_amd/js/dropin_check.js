@@ -5,7 +5,7 @@
 //
 // usage: node dropin_check.js '<cfg JSON>' <frames> <stepsPerFrame> [parcels]
 'use strict';
-const { GPUSolver, GPUSpray, buildSimSetup } = require('./gpu_solver.js');
+const { GPUSolver, GPUSpray, buildSimSetup, wallSeparation, summerfieldPoint } = require('./gpu_solver.js');
 
 const cfg = JSON.parse(process.argv[2]);
 const frames = parseInt(process.argv[3] || '5', 10);
@@ -68,12 +68,21 @@ for (let j = 0; j < setup.jExit; j++) {
   F += (rho * u * u + p - P.amb[3]) * dA;
 }
 const integ = solver.integrals(P);
+// render.js derive(V) for the Mach field (app.js V() fills the view), and the HUD
+// read-outs of app.js processWall (568) and the separation row (689)
+const V = { fieldIndex: 1, range: [0, 4], RaN: P.gas[0], Rs: P.Rs, chemK: P.chemK, Tc: 300, Ta: 300, schlGain: 1, dx: setup.dx };
+const der = solver.derive(V);
+let sumDerived = 0;
+for (let k = 0; k < der.length; k++) sumDerived += der[k];
+const wallRow = solver._readRect(solver.drawWallRow(), 0, 0, setup.Nx, 1);
+const sep = wallSeparation(setup, wallRow, P.amb[3] * 101325);
+const sfp = summerfieldPoint(setup.contour, 10e5, 1.4, 1e5);
 
 const out = {
   Nx: setup.Nx, Nr: setup.Nr, stepCount: solver.stepCount, stepInfo: Array.from(si),
   chamberA: Array.from(ch.A), sumA, sumB, finite, thrustJS: F * 101325, thrustDevice: integ.F,
   requested, polled, lastTag: lastAsync ? lastAsync.tag : null,
-  spray: solver.spray.stats(P), msPerStep: ms,
+  spray: solver.spray.stats(P), msPerStep: ms, sumDerived, wallRow: Array.from(wallRow), sep, summerfield: sfp,
 };
 solver.dispose();
 console.log(JSON.stringify(out));

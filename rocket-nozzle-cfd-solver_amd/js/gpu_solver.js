@@ -86,6 +86,12 @@ class GPUSolver {
   sync() { addon.sync(this._h); }
   setProbes(setup) { addon.setProbes(this._h, setup.iExit, setup.jExit, setup.iThroat, setup.iChamber, setup.dx); }
   integrals(P) { return addon.integrals(this._h, P); }
+  // render.js derive(V, second) (render.js:420-435): the derived field's Nx x Nr x
+  // (t, Z, Tn, M) values, returned instead of drawn into derivedFbo
+  derive(V, second = false) {
+    if (second) V = Object.assign({}, V, { fieldIndex: V.fieldIndex2, range: V.range2 });
+    return addon.derive(this._h, V, this.Nx, this.Nr);
+  }
   dispose() { if (this._h) { addon.destroy(this._h); this._h = null; } }
 }
 
@@ -111,7 +117,13 @@ class GPUSpray {
 function buildSimSetup(cfg) {
   const s = addon.buildSimSetup(cfg);
   s.initialState = () => ({ A: new Float32Array(s.init.A), B: new Float32Array(s.init.B) });
+  s.contour.handle = s.handle;
   return s;
 }
 
-module.exports = { GPUSolver, GPUSpray, buildSimSetup, deviceAvailable: addon.deviceAvailable };
+// HUD read-outs (simsetup.js:352-384) on the C++ setup; same arguments as the reference
+// (app.js:568 passes the setup, app.js:689 passes setup.contour)
+function wallSeparation(setup, row, paSi) { return addon.wallSeparation(setup.handle, row, paSi); }
+function summerfieldPoint(contour, Pc, g, pa) { return addon.summerfieldPoint(contour.handle, Pc, g, pa); }
+
+module.exports = { GPUSolver, GPUSpray, buildSimSetup, wallSeparation, summerfieldPoint, deviceAvailable: addon.deviceAvailable };

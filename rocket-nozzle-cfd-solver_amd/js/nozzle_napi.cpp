@@ -412,6 +412,60 @@ napi_value Integrals(napi_env env, napi_callback_info info) {
   set(env, o, "residual", mk_num(env, r.residual));
   return o;
 }
+// render.js derive(V) (render.js:420-435): V carries fieldIndex, range, RaN, Rs, chemK,
+// Tc, Ta, schlGain, dx as app.js V() builds it (app.js:375-382)
+napi_value Derived(napi_env env, napi_callback_info info) {
+  Args a;
+  if (!get_args(env, info, a) || a.n < 4) return throw_rc(env, -1, "derive");
+  nz_solver* s = solver_of(env, a.v[0]);
+  const int Nx = (int)num(env, a.v[2]), Nr = (int)num(env, a.v[3]);
+  if (!s || Nx <= 0 || Nr <= 0) return throw_rc(env, -1, "derive");
+  nz_derived_params d;
+  std::memset(&d, 0, sizeof(d));
+  napi_value V = a.v[1];
+  d.field = (int32_t)pnum(env, V, "fieldIndex");
+  parr(env, V, "range", d.range, 2);
+  d.RaN = (float)pnum(env, V, "RaN");
+  parr(env, V, "Rs", d.Rs, 3);
+  parr(env, V, "chemK", d.chemK, 4);
+  d.Tc = (float)pnum(env, V, "Tc");
+  d.Ta = (float)pnum(env, V, "Ta");
+  d.schlGain = (float)pnum(env, V, "schlGain");
+  d.dx = (float)pnum(env, V, "dx");
+  std::vector<float> out((size_t)Nx * Nr * 4);
+  if (int rc = nz_read_derived(s, &d, out.data())) return throw_rc(env, rc, "derive");
+  return new_f32(env, out.data(), out.size());
+}
+// wallSeparation(setup, row, paSi) (simsetup.js:352-373): object or null
+napi_value WallSeparation(napi_env env, napi_callback_info info) {
+  Args a;
+  if (!get_args(env, info, a) || a.n < 3) return throw_rc(env, -1, "wallSeparation");
+  nz_setup* st = setup_of(env, a.v[0]);
+  size_t n = 0;
+  const float* row = typed_data<float>(env, a.v[1], &n);
+  nz_setup_info in;
+  if (!st || !row || nz_setup_get_info(st, &in) || n < (size_t)in.Nx * 4) return throw_rc(env, -1, "wallSeparation");
+  nz_separation r;
+  if (nz_wall_separation(st, row, num(env, a.v[2]), &r) != 1) return mk_null(env);
+  napi_value o;
+  napi_create_object(env, &o);
+  set(env, o, "x", mk_num(env, r.x));
+  set(env, o, "frac", mk_num(env, r.frac));
+  set(env, o, "eps", mk_num(env, r.eps));
+  set(env, o, "pRatio", mk_num(env, r.pRatio));
+  set(env, o, "Msep", mk_num(env, r.Msep));
+  return o;
+}
+// summerfieldPoint(contour, Pc, g, pa) (simsetup.js:375-384): fraction or null
+napi_value SummerfieldPoint(napi_env env, napi_callback_info info) {
+  Args a;
+  if (!get_args(env, info, a) || a.n < 4) return throw_rc(env, -1, "summerfieldPoint");
+  nz_setup* st = setup_of(env, a.v[0]);
+  if (!st) return throw_rc(env, -1, "summerfieldPoint");
+  double f = 0;
+  if (nz_summerfield_point(st, num(env, a.v[1]), num(env, a.v[2]), num(env, a.v[3]), &f) != 1) return mk_null(env);
+  return mk_num(env, f);
+}
 napi_value DeviceAvailable(napi_env env, napi_callback_info) { return mk_bool(env, nz_device_available() == 1); }
 
 // ------------------------------------------------------------------ host setup (buildSimSetup)
@@ -511,7 +565,18 @@ napi_value BuildSetup(napi_env env, napi_callback_info info) {
   set(env, atm, "p", mk_num(env, in.atmP));
   set(env, atm, "T", mk_num(env, in.atmT));
   set(env, out, "atm", atm);
-  nz_setup_free(s);
+  napi_value ct;
+  napi_create_object(env, &ct);
+  const std::pair<const char*, double> cf[] = {{"Rt", in.Rt}, {"Re", in.Re}, {"Rc", in.Rc}, {"Lc", in.Lc}, {"xt", in.xt},
+                                               {"xe", in.xe}, {"eps", in.eps}, {"thn", in.thn}, {"the", in.the}};
+  for (const auto& kv : cf) set(env, ct, kv.first, mk_num(env, kv.second));
+  set(env, out, "contour", ct);
+  // the setup stays alive for the HUD read-outs (wallSeparation, summerfieldPoint) and is
+  // freed with the JS object that holds it
+  napi_value ext;
+  NAPI_OK(napi_create_external(env, s, [](napi_env, void* p, void*) { nz_setup_free(static_cast<nz_setup*>(p)); },
+                               nullptr, &ext));
+  set(env, out, "handle", ext);
   return out;
 }
 
@@ -523,6 +588,7 @@ napi_value Init(napi_env env, napi_value exports) {
       {"sync", Sync}, {"asyncBegin", AsyncBegin}, {"asyncPoll", AsyncPoll}, {"asyncCancel", AsyncCancel},
       {"sprayInjector", SprayInjector}, {"sprayReset", SprayReset}, {"sprayCount", SprayCount}, {"sprayStats", SprayStats},
       {"setProbes", Probes}, {"integrals", Integrals}, {"deviceAvailable", DeviceAvailable}, {"buildSimSetup", BuildSetup},
+      {"derive", Derived}, {"wallSeparation", WallSeparation}, {"summerfieldPoint", SummerfieldPoint},
   };
   for (const Fn& f : fns) {
     napi_value fn;

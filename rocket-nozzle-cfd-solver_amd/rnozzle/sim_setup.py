@@ -105,6 +105,25 @@ class SimSetup:
     def probes(self):
         return abi.NzProbes(self.iExit, self.jExit, self.iThroat, self.iChamber, self.dx)
 
+    def wall_separation(self, row, paSi):
+        """wallSeparation(setup, row, paSi) (simsetup.js:352-373) on the near-wall row
+        (Nx x (u, v, p, c), as nz_draw_wall_row leaves it): a dict, or None while attached."""
+        row = np.ascontiguousarray(row, np.float32)
+        if row.size < self.Nx * 4:
+            raise ValueError(f"wall row needs {self.Nx * 4} floats, got {row.size}")
+        out = abi.NzSeparation()
+        if lib().nz_wall_separation(self._h, _fptr(row), float(paSi), C.byref(out)) != 1:
+            return None
+        return {n: getattr(out, n) for n, _ in abi.NzSeparation._fields_}
+
+    def summerfield_point(self, PcSi, gamma, paSi):
+        """summerfieldPoint(contour, Pc, g, pa) (simsetup.js:375-384): fraction of the bell,
+        or None if the 1-D wall pressure never drops below 0.4 pa."""
+        frac = C.c_double()
+        if lib().nz_summerfield_point(self._h, float(PcSi), float(gamma), float(paSi), C.byref(frac)) != 1:
+            return None
+        return frac.value
+
     def __del__(self):
         try:
             if getattr(self, "_h", None):

@@ -43,3 +43,23 @@ def test_wall_separation_and_summerfield(name):
             assert got == 0
         else:
             assert got == 1 and frac.value == sf["frac"]
+
+
+def test_wrappers_follow_the_reference_return_shape():
+    """SimSetup.wall_separation / summerfield_point return what the reference's functions
+    return: an object (dict) or null (None)."""
+    name = sorted(GOLD)[0]
+    case = GOLD[name]
+    s = SimSetup(default_config(**case["cfg"]))
+    for r in case["rows"]:
+        got = s.wall_separation(np.asarray(r["row"], np.float32), r["pa"])
+        if r["sep"] is None:
+            assert got is None
+        else:
+            assert set(got) == {"x", "frac", "eps", "pRatio", "Msep"}
+            assert got["frac"] == pytest.approx(r["sep"]["frac"], rel=1e-12, abs=1e-15)
+    for sf in case["summerfield"]:
+        got = s.summerfield_point(sf["Pc"], sf["g"], sf["pa"])
+        assert got == sf["frac"]
+    with pytest.raises(ValueError):
+        s.wall_separation(np.zeros(3, np.float32), 101325.0)

@@ -49,6 +49,33 @@ def node_setup(cfg):
 
 
 @needs_node
+def test_hud_readouts_in_js_match_reference_outputs():
+    """wallSeparation / summerfieldPoint through the addon against the reference's own
+    outputs (tests/golden/hud_golden.json, written by tests/golden/make_hud_golden.js)."""
+    gold_path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "hud_golden.json")
+    code = ("const m=require(process.argv[1]);const G=JSON.parse(require('fs').readFileSync(process.argv[2]));"
+            "const out={};for(const n of Object.keys(G)){const c=G[n];const s=m.buildSimSetup(c.cfg);"
+            "out[n]={sep:c.rows.map(r=>m.wallSeparation(s,Float32Array.from(r.row),r.pa)),"
+            "sf:c.summerfield.map(f=>m.summerfieldPoint(s.contour,f.Pc,f.g,f.pa)),Lc:s.contour.Lc}}"
+            "console.log(JSON.stringify(out));")
+    r = subprocess.run([NODE, "-e", code, os.path.join(PKG, "js", "gpu_solver.js"), gold_path],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr[-3000:]
+    got = json.loads(r.stdout)
+    gold = json.load(open(gold_path))
+    for name, case in gold.items():
+        g = got[name]
+        assert g["Lc"] > 0
+        for row, sep in zip(case["rows"], g["sep"]):
+            if row["sep"] is None:
+                assert sep is None, (name, row["kind"])
+            else:
+                for k in ("x", "frac", "eps", "pRatio", "Msep"):
+                    assert sep[k] == pytest.approx(row["sep"][k], rel=1e-12, abs=1e-15), (name, row["kind"], k)
+        assert g["sf"] == [f["frac"] for f in case["summerfield"]]
+
+
+@needs_node
 def test_constructor_throws_like_reference_without_device():
     code = ("const m=require(process.argv[1]);if(m.deviceAvailable()){console.log('device');process.exit(0)}"
             "try{new m.GPUSolver(null);console.log('no-throw')}catch(e){console.log('threw:'+(e instanceof Error))}")
@@ -72,4 +99,15 @@ def test_dropin_sequence_matches_c_abi_path():
     a = g.readAll()
     assert float(np.sum(a["A"].astype(np.float64))) == pytest.approx(out["sumA"], rel=1e-12)
     assert list(g.readStepInfo()) == pytest.approx(out["stepInfo"], rel=1e-7)
+    # derive(V) and the HUD read-outs agree with the Python host on the same state
+    der = g.derived(1, (0.0, 4.0), s.P, 300.0, 300.0, 1.0, s.dx)
+    assert float(np.sum(der.astype(np.float64))) == pytest.approx(out["sumDerived"], rel=1e-12)
+    g.drawWallRow()
+    row = g._readRect("wallRow", 0, 0, s.Nx, 1)
+    assert np.array_equal(row, np.asarray(out["wallRow"], np.float32))
+    sep = s.wall_separation(row, float(s.P.amb[3]) * 101325)
+    assert (sep is None) == (out["sep"] is None)
+    if sep is not None:
+        assert sep["frac"] == pytest.approx(out["sep"]["frac"], rel=1e-12)
+    assert s.summerfield_point(10e5, 1.4, 1e5) == out["summerfield"]
     print("drop-in ms/step (self-test loop):", out["msPerStep"])
