@@ -106,6 +106,8 @@ def plain_one_at_a_time(items):
 
 def tune_shape(setup, args):
     """Tile shape for a batch: the autotuner's pick on its largest solve (0 with --no-tune)."""
+    if args.shape >= 0:
+        return args.shape
     if args.no_tune:
         return 0
     from rnozzle.solver import GPUSolver, load_setup
@@ -133,7 +135,11 @@ def bench_batch(args):
     ids = [b.submit(s, t_end_s=configs.BATCH_T_END, res_tol=configs.BATCH_RES_TOL, min_steps=100) for s in setups]
     t0 = time.perf_counter()
     # the autotuner runs inside the timed region: its trial steps advance the solves
-    shape, tune_us = b.autotune(20) if not args.no_tune else (0, None)
+    if args.shape >= 0:
+        b.set_tile_shape(args.shape)
+        shape, tune_us = args.shape, None
+    else:
+        shape, tune_us = b.autotune(20) if not args.no_tune else (0, None)
     runs = 0
     while True:
         r, q = b.run(args.chunk)
@@ -268,6 +274,7 @@ def main():
     ap.add_argument("--set", action="append", default=[], help="config override key=value (e.g. wallModel=stair)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="keep the default tile shape")
+    ap.add_argument("--shape", type=int, default=-1, help="fix the tile shape id (no autotuning); for profiles")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--instances", type=int, default=64, help="batch: solves in the set")
     ap.add_argument("--seed", type=int, default=20240501)
@@ -321,7 +328,9 @@ def main():
     # each trial) and keeps the fastest; outside the timed region
     from rnozzle.solver import shape_name, tile_shapes
     tune_us = None
-    if not args.no_tune:
+    if args.shape >= 0:
+        g.setTileShape(args.shape)
+    elif not args.no_tune:
         _, tune_us = g.autotune(s.P, steps=20)
     shape = g.tileShape()
     # timed region: K steps enqueued back to back on the solver's stream

@@ -832,7 +832,8 @@ struct Oracle {
           }
           case 7: v = Z; break;
           case 9: {
-            const T m0 = T2[((size_t)j * Nx + i) * 4], m3 = T2[((size_t)j * Nx + i) * 4 + 3];
+            // vis.T2 is an RGBA32F texture (render.js:426): its texels hold float values
+            const T m0 = T((float)T2[((size_t)j * Nx + i) * 4]), m3 = T((float)T2[((size_t)j * Nx + i) * 4 + 3]);
             v = m3 > T(0) ? std::log(std::max(m0 / m3, T(1))) / std::log(T(10)) : T(0);
             break;
           }
@@ -840,7 +841,7 @@ struct Oracle {
           case 11: v = F; break;
           case 12: v = O; break;
           case 13: {
-            const T m0 = T2[((size_t)j * Nx + i) * 4], m2 = T2[((size_t)j * Nx + i) * 4 + 2];
+            const T m0 = T((float)T2[((size_t)j * Nx + i) * 4]), m2 = T((float)T2[((size_t)j * Nx + i) * 4 + 2]);
             const T avail = std::min(F, (O + T(0.233f) * std::max(T(1) - Z, T(0))) / Tp<T>(d.chemK[1]));
             const T kin = Tp<T>(d.chemK[2]) * p * std::exp(T(-9000) / std::max(TT, T(200)));
             v = std::log(T(1) + T(1e5f) * avail * std::min(std::max(m2, T(4) * m0 / rho), kin)) / std::log(T(10));
