@@ -25,6 +25,16 @@ __device__ __forceinline__ int find_inst(const DevInst* __restrict__ I, int n, i
   return lo;
 }
 
+// Tile of a workgroup. The dispatcher deals workgroups round-robin over the 8 XCDs
+// (workgroup b runs on XCD b % 8), and each XCD has its own L2. Workgroup b takes tile
+// start(b % 8) + b / 8, so every XCD works on one contiguous run of tiles (whole tile
+// rows of the grid) and the halo rows its tiles share are fetched into one L2 once.
+__device__ __forceinline__ int xcd_tile(int b, int nblk) {
+  constexpr int NXCD = 8;
+  const int x = b % NXCD, q = nblk / NXCD, r = nblk % NXCD;
+  return x * q + min(x, r) + b / NXCD;
+}
+
 // FS_STEPINFO (solver.js:404-409): dt from the wave-speed max of the current state,
 // in the arithmetic type of the state
 template <typename R>
@@ -209,7 +219,7 @@ __global__ void __launch_bounds__(TX* TY, (STAGE == 2 && PHYS) ? 1 : W) k_stage(
   using S = StageSmem<R, TX, TY, STAGE == 2 && PHYS>;
   __shared__ S sm;
   const int tid = threadIdx.x;
-  const int t = blockIdx.x;
+  const int t = xcd_tile(blockIdx.x, gridDim.x);
   const int ii = find_inst(insts, ninst, t);
   const DevInst& I = insts[ii];
   const Consts& c = I.c;
