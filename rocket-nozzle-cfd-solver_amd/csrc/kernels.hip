@@ -59,7 +59,6 @@ __global__ void __launch_bounds__(256) k_finalize(const DevInst* __restrict__ in
     if (J.c.tEnd > 0.f && s->last[2] >= (double)J.c.tEnd) s->flags |= 1u | 4u;
     if (J.c.resTol > 0.f && s->steps >= J.c.minSteps && s->resLast <= (double)J.c.resTol) s->flags |= 1u | 2u;
     if (J.c.maxSteps > 0 && s->steps >= J.c.maxSteps) s->flags |= 1u | 8u;
-    if (J.c.maxSteps > 0 && s->steps >= J.c.maxSteps) s->flags |= 1u | 8u;
   }
 }
 

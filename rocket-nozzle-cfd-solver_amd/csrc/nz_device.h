@@ -147,6 +147,14 @@ __device__ __forceinline__ double rc(double b) { return 1.0 / b; }
 __device__ __forceinline__ float sq(float x) { return __builtin_amdgcn_sqrtf(x); }
 __device__ __forceinline__ double sq(double x) { return sqrt(x); }
 
+// Power and exponential in the state's precision. FP32 uses the hardware base-2
+// exponential and logarithm (v_exp_f32 / v_log_f32), which is how GLSL ES 3.00 defines
+// pow(x, y) = exp2(y log2 x) (x > 0; x = 0 gives 0 for y > 0); FP64 keeps the library.
+__device__ __forceinline__ float powR(float x, float y) { return __builtin_amdgcn_exp2f(y * __builtin_amdgcn_logf(x)); }
+__device__ __forceinline__ double powR(double x, double y) { return pow(x, y); }
+__device__ __forceinline__ float expR(float x) { return __builtin_amdgcn_exp2f(x * 1.44269504088896341f); }
+__device__ __forceinline__ double expR(double x) { return exp(x); }
+
 template <typename R>
 __device__ __forceinline__ R clampR(R x, R lo, R hi) { return fmin(fmax(x, lo), hi); }
 template <typename R>
@@ -237,7 +245,7 @@ __device__ Prim<R> prim_from(const DevInst& I, const CellRaw<R>& s, int i, int j
       const R ut = ubar * smoothstepR(R(1), R(2), P0 / fmax(R(c.pStart), R(1e-6)));
       const R u = mixR(ui, ut, R(c.inlet[1]));
       const R T = T0 - u * u * (g - R(1)) / (R(2) * g * Rg);
-      const R p = P0 * pow(T / T0, g * G);
+      const R p = P0 * powR(T / T0, g * G);
       q.r = p / (Rg * T); q.u = u; q.v = R(0); q.p = p;
       q.Z = R(1); q.F = R(1); q.G = G; q.O = R(0);
     }
@@ -410,7 +418,7 @@ __device__ __forceinline__ R wall_pressure(const Prim<R>& q, R un) {
     return p + rho * un * (a + sq(a * a + c * c));
   }
   const R f = fmax(R(1) + dv(R(0.5) * (g - R(1)) * un, c), R(0));
-  return p * pow(f, dv(R(2) * g, g - R(1)));
+  return p * powR(f, dv(R(2) * g, g - R(1)));
 }
 
 // Sutherland viscosity and mixture gas constant (solver.js:448-449)

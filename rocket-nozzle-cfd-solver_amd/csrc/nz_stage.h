@@ -26,13 +26,22 @@ __device__ __forceinline__ int find_inst(const DevInst* __restrict__ I, int n, i
 }
 
 // Tile of a workgroup. The dispatcher deals workgroups round-robin over the 8 XCDs
-// (workgroup b runs on XCD b % 8), and each XCD has its own L2. Workgroup b takes tile
-// start(b % 8) + b / 8, so every XCD works on one contiguous run of tiles (whole tile
-// rows of the grid) and the halo rows its tiles share are fetched into one L2 once.
-__device__ __forceinline__ int xcd_tile(int b, int nblk) {
+// (workgroup b runs on XCD b % 8), and each XCD has its own L2. The tiles are cut into
+// chunks of `chunk` consecutive tiles (whole tile rows), dealt round-robin to the XCDs:
+// an XCD's workgroups take its chunks in order, so the halos shared inside a chunk are
+// fetched into one L2 once, while the costly tiles (walls, merged cells) of one region
+// of the nozzle spread over several XCDs. Tiles past the last whole round of chunks go
+// to the XCDs in contiguous runs. Bijective on [0, nblk).
+#ifndef NZ_XCD_CHUNK_ROWS
+#define NZ_XCD_CHUNK_ROWS 2
+#endif
+__device__ __forceinline__ int xcd_tile(int b, int nblk, int chunk) {
   constexpr int NXCD = 8;
-  const int x = b % NXCD, q = nblk / NXCD, r = nblk % NXCD;
-  return x * q + min(x, r) + b / NXCD;
+  const int x = b % NXCD, l = b / NXCD;
+  const int per = (nblk / (NXCD * chunk)) * chunk;  // chunked tiles per XCD
+  if (l < per) return ((l / chunk) * NXCD + x) * chunk + l % chunk;
+  const int rest = nblk - per * NXCD, q = rest / NXCD, r = rest % NXCD;
+  return per * NXCD + x * q + min(x, r) + (l - per);
 }
 
 // FS_STEPINFO (solver.js:404-409): dt from the wave-speed max of the current state,
@@ -219,7 +228,7 @@ __global__ void __launch_bounds__(TX* TY, (STAGE == 2 && PHYS) ? 1 : W) k_stage(
   using S = StageSmem<R, TX, TY, STAGE == 2 && PHYS>;
   __shared__ S sm;
   const int tid = threadIdx.x;
-  const int t = xcd_tile(blockIdx.x, gridDim.x);
+  const int t = xcd_tile(blockIdx.x, gridDim.x, max(insts[0].tilesX * NZ_XCD_CHUNK_ROWS, 1));
   const int ii = find_inst(insts, ninst, t);
   const DevInst& I = insts[ii];
   const Consts& c = I.c;
@@ -621,7 +630,7 @@ __global__ void __launch_bounds__(TX* TY, (STAGE == 2 && PHYS) ? 1 : W) k_stage(
             const R dz = R(2) * clampR(R(0.5) * V * iwl, R(0.25), R(1));
             const R nzv = dv(mus, rs * dz);
             R tw = up <= R(69.7) * nzv ? dv(R(2) * mus * up, dz)
-                                       : rs * pow(R(7.187) * pow(nzv, R(1.142857)) + R(0.13769) * pow(nzv, R(0.142857)) * up, R(1.75));
+                                       : rs * powR(R(7.187) * powR(nzv, R(1.142857)) + R(0.13769) * powR(nzv, R(0.142857)) * up, R(1.75));
             const R iwdt = rc(fmax(wl * dt, R(1e-9)));
             tw = fmin(tw, R(0.3) * rho * up * V * iwdt);
             if (up > R(1e-6)) { const R f = dv(tw * wl, up); L[1] -= f * utx; L[2] -= f * utr; }
@@ -667,7 +676,7 @@ __global__ void __launch_bounds__(TX* TY, (STAGE == 2 && PHYS) ? 1 : W) k_stage(
                                 : fmin(Fv, fmax(R(1) - Z - s * (Z - Fv), R(0)) / s);
           if (avail > R(1e-5)) {
             const R TK = Tg * R(c.gas[2]);
-            const R kin = R(c.chemK[2]) * p * exp(R(-9000) / TK) * (R(1) - smoothstepR(R(c.cap[0]), R(c.cap[1]), TK));
+            const R kin = R(c.chemK[2]) * p * expR(R(-9000) / TK) * (R(1) - smoothstepR(R(c.cap[0]), R(c.cap[1]), TK));
             const R mixr = c.inj ? fmax(T2h[2], R(4) * T2h[0] / rho) : T2h[2];
             const R burn = avail * fmin(fmin(mixr, kin) * dt, R(0.5));
             UY[1] -= rho * burn;
