@@ -87,10 +87,14 @@ __device__ __forceinline__ double block_sum(double v, double* red) {
 // One face's inviscid flux times |area| (FS_FLUX_X/R for the face, as memberFlux
 // weights it), with primitives from `getP(i, j)`: the LDS tile, or the tile with a
 // global fall-back for faces of merged cells that lie outside it; the signed area from
-// `getA(dirR, i, j)` (LDS for the tile's own faces, else global memory).
-template <typename R, class GetP, class GetA>
-__device__ __forceinline__ void face_flux_w(const DevInst& I, bool dirR, int fi, int fj, GetP getP, GetA getA, R* F,
+// `getA(dirR, i, j)` (LDS for the tile's own faces, else global memory). DIR 0 / 1 fixes
+// the direction at compile time (x / r faces; the `dirR` argument is then ignored), so
+// the cell addresses are constant offsets and the velocity swap costs nothing; DIR -1
+// takes it from `dirR`.
+template <typename R, int DIR = -1, class GetP, class GetA>
+__device__ __forceinline__ void face_flux_w(const DevInst& I, bool dirR_, int fi, int fj, GetP getP, GetA getA, R* F,
                                             R* FY, R& FO) {
+  const bool dirR = DIR < 0 ? dirR_ : DIR == 1;
   for (int q = 0; q < 4; ++q) { F[q] = R(0); FY[q] = R(0); }
   FO = R(0);
   const int Nx = I.Nx, Nr = I.Nr;
@@ -126,9 +130,10 @@ __device__ __forceinline__ void face_flux_w(const DevInst& I, bool dirR, int fi,
 
 // One face's viscous flux times |area| (FS_VFLUX_X/R); getT(i, j, T1, T2) gives the
 // per-cell gradients / viscosities of an interior fluid cell.
-template <typename R, class GetP, class GetT, class GetA>
-__device__ __forceinline__ void vface_w(const DevInst& I, bool dirR, int fi, int fj, GetP getP, GetT getT, GetA getA, R* F,
+template <typename R, int DIR = -1, class GetP, class GetT, class GetA>
+__device__ __forceinline__ void vface_w(const DevInst& I, bool dirR_, int fi, int fj, GetP getP, GetT getT, GetA getA, R* F,
                                         R* FY) {
+  const bool dirR = DIR < 0 ? dirR_ : DIR == 1;  // as in face_flux_w
   for (int q = 0; q < 4; ++q) F[q] = R(0);
   for (int q = 0; q < 3; ++q) FY[q] = R(0);
   const int Nx = I.Nx, Nr = I.Nr;
@@ -374,24 +379,34 @@ __global__ void __launch_bounds__(TX* TY, (STAGE == 2 && PHYS) ? 1 : W) k_stage(
     };
     __syncthreads();
 
-    // ---- 2. each face flux once (FS_FLUX_X / FS_FLUX_R), times |area|; extra faces last
-    for (int f = tid; f < S::NFX + S::NFR + nEx; f += NT) {
+    // ---- 2. each face flux once (FS_FLUX_X / FS_FLUX_R), times |area|: the west and
+    // the south face of every tile cell (one each per thread, direction fixed at compile
+    // time), then the tile's east column and north row of faces and the extra faces
+    auto putX = [&](int fx, int fy, const R* F, const R* FY, R FO) {
+      for (int q = 0; q < 4; ++q) { sm.u.inv.Fx[q][fy][fx] = F[q]; sm.u.inv.Fx[4 + q][fy][fx] = FY[q]; }
+      sm.u.inv.Fx[8][fy][fx] = FO;
+    };
+    auto putR = [&](int fx, int fy, const R* F, const R* FY, R FO) {
+      for (int q = 0; q < 4; ++q) { sm.u.inv.Fr[q][fy][fx] = F[q]; sm.u.inv.Fr[4 + q][fy][fx] = FY[q]; }
+      sm.u.inv.Fr[8][fy][fx] = FO;
+    };
+    {
       R F[4], FY[4], FO;
-      if (f < S::NFX + S::NFR) {
-        const bool dirR = f >= S::NFX;
-        const int g = dirR ? f - S::NFX : f;
-        const int fx = dirR ? g % TX : g % (TX + 1);
-        const int fy = dirR ? g / TX : g / (TX + 1);
+      face_flux_w<R, 0>(I, false, i0 + tx, j0 + ty, tileP, tileA, F, FY, FO);
+      putX(tx, ty, F, FY, FO);
+      face_flux_w<R, 1>(I, true, i0 + tx, j0 + ty, tileP, tileA, F, FY, FO);
+      putR(tx, ty, F, FY, FO);
+    }
+    for (int f = tid; f < TY + TX + nEx; f += NT) {
+      R F[4], FY[4], FO;
+      if (f < TY + TX) {
+        const bool dirR = f >= TY;
+        const int fx = dirR ? f - TY : TX, fy = dirR ? TY : f;
         face_flux_w<R>(I, dirR, i0 + fx, j0 + fy, tileP, tileA, F, FY, FO);
-        if (dirR) {
-          for (int q = 0; q < 4; ++q) { sm.u.inv.Fr[q][fy][fx] = F[q]; sm.u.inv.Fr[4 + q][fy][fx] = FY[q]; }
-          sm.u.inv.Fr[8][fy][fx] = FO;
-        } else {
-          for (int q = 0; q < 4; ++q) { sm.u.inv.Fx[q][fy][fx] = F[q]; sm.u.inv.Fx[4 + q][fy][fx] = FY[q]; }
-          sm.u.inv.Fx[8][fy][fx] = FO;
-        }
+        if (dirR) putR(fx, fy, F, FY, FO);
+        else putX(fx, fy, F, FY, FO);
       } else {
-        const int e = f - S::NFX - S::NFR;
+        const int e = f - TY - TX;
         const uint2 ef = sm.ex[e];
         face_flux_w<R>(I, ex_dir(ef), ex_i(ef), ex_j(ef), anyP, globA, F, FY, FO);
         for (int q = 0; q < 4; ++q) { gst(exFlux, e * 16 + q, F[q]); gst(exFlux, e * 16 + 4 + q, FY[q]); }
@@ -553,23 +568,32 @@ __global__ void __launch_bounds__(TX* TY, (STAGE == 2 && PHYS) ? 1 : W) k_stage(
         for (int q = 0; q < 4; ++q) { T1[q] = sm.u.vis.T[q][rj][ri]; T2[q] = sm.u.vis.T[4 + q][rj][ri]; }
       };
       if (c.visc) {
-        for (int f = tid; f < S::NFX + S::NFR + nEx; f += NT) {
+        // as the inviscid faces: west and south face of each cell, then the rest
+        auto putVX = [&](int fx, int fy, const R* F, const R* FY) {
+          for (int q = 0; q < 4; ++q) sm.u.vis.Vx[q][fy][fx] = F[q];
+          for (int q = 0; q < 3; ++q) sm.u.vis.Vx[4 + q][fy][fx] = FY[q];
+        };
+        auto putVR = [&](int fx, int fy, const R* F, const R* FY) {
+          for (int q = 0; q < 4; ++q) sm.u.vis.Vr[q][fy][fx] = F[q];
+          for (int q = 0; q < 3; ++q) sm.u.vis.Vr[4 + q][fy][fx] = FY[q];
+        };
+        {
           R F[4], FY[3];
-          if (f < S::NFX + S::NFR) {
-            const bool dirR = f >= S::NFX;
-            const int g = dirR ? f - S::NFX : f;
-            const int fx = dirR ? g % TX : g % (TX + 1);
-            const int fy = dirR ? g / TX : g / (TX + 1);
+          vface_w<R, 0>(I, false, i0 + tx, j0 + ty, tileP, tileT, tileA, F, FY);
+          putVX(tx, ty, F, FY);
+          vface_w<R, 1>(I, true, i0 + tx, j0 + ty, tileP, tileT, tileA, F, FY);
+          putVR(tx, ty, F, FY);
+        }
+        for (int f = tid; f < TY + TX + nEx; f += NT) {
+          R F[4], FY[3];
+          if (f < TY + TX) {
+            const bool dirR = f >= TY;
+            const int fx = dirR ? f - TY : TX, fy = dirR ? TY : f;
             vface_w<R>(I, dirR, i0 + fx, j0 + fy, tileP, tileT, tileA, F, FY);
-            if (dirR) {
-              for (int q = 0; q < 4; ++q) sm.u.vis.Vr[q][fy][fx] = F[q];
-              for (int q = 0; q < 3; ++q) sm.u.vis.Vr[4 + q][fy][fx] = FY[q];
-            } else {
-              for (int q = 0; q < 4; ++q) sm.u.vis.Vx[q][fy][fx] = F[q];
-              for (int q = 0; q < 3; ++q) sm.u.vis.Vx[4 + q][fy][fx] = FY[q];
-            }
+            if (dirR) putVR(fx, fy, F, FY);
+            else putVX(fx, fy, F, FY);
           } else {
-            const int e = f - S::NFX - S::NFR;
+            const int e = f - TY - TX;
             const uint2 ef = sm.ex[e];
             vface_w<R>(I, ex_dir(ef), ex_i(ef), ex_j(ef), anyP, anyT, globA, F, FY);
             for (int q = 0; q < 4; ++q) gst(exFlux, e * 16 + 9 + q, F[q]);

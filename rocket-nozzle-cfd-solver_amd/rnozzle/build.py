@@ -18,8 +18,10 @@ ARCH = "gfx950"
 HIP_SOURCES = ["kernels.hip", "instance.hip", "runtime.hip", "batch.hip"]
 CPP_SOURCES = ["setup.cpp", "errors.cpp"]
 
-# device code: no packed-FP32 instructions (they are avoided on this platform)
-DEVICE_FLAGS = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+# device code: no packed-FP32 instructions (they are avoided on this platform); small
+# per-thread arrays stay in private memory rather than being moved into LDS, whose size
+# sets how many stage workgroups share a CU
+DEVICE_FLAGS = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-mllvm", "-disable-promote-alloca-to-lds"]
 
 
 def stage_shapes():
