@@ -95,6 +95,39 @@ template <typename R, int DIR = -1, class GetP, class GetA>
 __device__ __forceinline__ void face_flux_w(const DevInst& I, bool dirR_, int fi, int fj, GetP getP, GetA getA, R* F,
                                             R* FY, R& FO) {
   const bool dirR = DIR < 0 ? dirR_ : DIR == 1;
+  if constexpr (DIR >= 0) {
+    // a face of the LDS tile: every cell it reads is in the tile and its area is 0
+    // outside the domain, so the flux is formed unconditionally and then kept or set
+    // to zero (no divergent early exits; a dropped face's arithmetic may be non-finite
+    // and is discarded by the selects)
+    constexpr int dx = DIR == 1 ? 0 : 1, dy = DIR == 1 ? 1 : 0, tx = dy, ty = dx;
+    const float area = getA(dirR, fi, fj);
+    Prim<R> c0 = getP(fi - 2 * dx, fj - 2 * dy), c1 = getP(fi - dx, fj - dy);
+    Prim<R> c2 = getP(fi, fj), c3 = getP(fi + dx, fj + dy);
+    const bool s1 = c1.r < R(0), s2 = c2.r < R(0);
+    const bool keep = area != 0.f && !(DIR == 1 && fj == 0) && !(s1 && s2);
+    R hllw = R(0);
+    if (I.c.hllBlend > 0.f) {
+      const Prim<R> a1 = getP(fi - dx + tx, fj - dy + ty), b1 = getP(fi - dx - tx, fj - dy - ty);
+      const Prim<R> a2 = getP(fi + tx, fj + ty), b2 = getP(fi - tx, fj - ty);
+      const R w = hll_weight<R>(c1.p, c2.p, a1.r < R(0) ? c1.p : a1.p, b1.r < R(0) ? c1.p : b1.p,
+                                a2.r < R(0) ? c2.p : a2.p, b2.r < R(0) ? c2.p : b2.p, R(I.c.hllBlend));
+      hllw = (s1 || s2) ? R(0) : w;
+    }
+    if (DIR == 1) {
+      R t;
+      t = c0.u; c0.u = c0.v; c0.v = t;
+      t = c1.u; c1.u = c1.v; c1.v = t;
+      t = c2.u; c2.u = c2.v; c2.v = t;
+      t = c3.u; c3.u = c3.v; c3.v = t;
+    }
+    face_flux<R>(c0, c1, c2, c3, hllw, I.c.order2 && area > 0.f, F, FY, FO);
+    if (DIR == 1) { const R t = F[1]; F[1] = F[2]; F[2] = t; }
+    const R a = R(fabsf(area));
+    for (int q = 0; q < 4; ++q) { F[q] = keep ? F[q] * a : R(0); FY[q] = keep ? FY[q] * a : R(0); }
+    FO = keep ? FO * a : R(0);
+    return;
+  }
   for (int q = 0; q < 4; ++q) { F[q] = R(0); FY[q] = R(0); }
   FO = R(0);
   const int Nx = I.Nx, Nr = I.Nr;
