@@ -221,13 +221,37 @@ def test_injector_fp64_logic():
 
 
 def test_rescale_ambient(cold):
+    """rescaleAmbient mid-run, then 20 steps of the start-up transient. Rounding grows
+    fast here: the FP32 oracle itself sits about 1e-4 (rho) to 1.5e-3 (v) from the FP64
+    oracle, so each channel's bar is 1e-4 or twice that FP32 rounding noise, whichever is
+    larger, and the GPU's FP32 result must be no further from the FP64 oracle than twice
+    the FP32 oracle is."""
     g, o = run_pair(cold, 60)
     P2 = cold.compute_params(cold.cfg.Pc, cold.info.atmP)
-    g.rescaleAmbient(0.8, 0.95, 0.1, P2)
-    o.rescale(0.8, 0.95, 0.1, P2)
-    g.step(20, P2)
-    o.step(20, P2)
-    compare_state(g, o, 1e-4, "after rescaleAmbient")
+    o64 = orc.Oracle(cold.Nx, cold.Nr, 64)
+    A0, B0 = cold.initial_state()
+    o64.set_geometry(cold.mask, cold.geoA, cold.geoG, cold.geoM, cold.jWall)
+    o64.set_state(A0, B0)
+    o64.step(60, cold.P)
+    for x in (g, o, o64):
+        (x.rescaleAmbient if x is g else x.rescale)(0.8, 0.95, 0.1, P2)
+        x.step(20, P2)
+    G = g.readAll()
+    pg = prims(G["A"], G["B"])
+    po = prims(*o.state())
+    a64 = o64.state_f64()
+    p64 = prims(a64[0], a64[1])
+    po32 = prims(*o.state_f64())
+    errs = {k: rel_l2(pg[k], po[k]) for k in pg}
+    noise = {k: rel_l2(po32[k], p64[k]) for k in pg}
+    to64 = {k: rel_l2(pg[k], p64[k]) for k in pg}
+    say("[parity] after rescaleAmbient: rel-L2 " + " ".join(f"{k}={v:.2e}" for k, v in errs.items()))
+    say("[parity] after rescaleAmbient: oracle FP32 vs FP64 " + " ".join(f"{k}={v:.2e}" for k, v in noise.items()))
+    say("[parity] after rescaleAmbient: GPU FP32 vs oracle FP64 " + " ".join(f"{k}={v:.2e}" for k, v in to64.items()))
+    for k in pg:
+        assert np.isfinite(pg[k]).all()
+        assert errs[k] <= max(1e-4, 2 * noise[k]), (k, errs[k], noise[k])
+        assert to64[k] <= max(1e-4, 2 * noise[k]), (k, to64[k], noise[k])
 
 
 def test_readbacks_and_async(cold):
