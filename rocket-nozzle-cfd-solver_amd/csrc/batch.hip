@@ -168,11 +168,15 @@ int upload_table(nz_batch* b) {
 int enqueue_steps(nz_batch* b, int steps) {
   const bool fp64 = b->prec == 64;
   const int n = (int)b->res.size();
+  // a resident with more tile partials than stage 1 folds itself needs the close kernel
+  // after every step; else the close at the end makes the step state current
+  bool closeEach = false;
+  for (const auto& r : b->res) closeEach |= (unsigned)(r->I.tilesX * r->I.tilesY) > nzk::FUSED_CLOSE_MAX;
   for (int k = 0; k < steps; ++k) {
     HIPCHK(nzk::launch_stage(1, false, fp64, b->shape, b->dtab, n, b->ntiles, b->stream), "batch stage 1");
     if (b->npblk) HIPCHK(nzk::launch_spray(fp64, b->dtab, n, b->npblk, b->stream), "batch spray");
     HIPCHK(nzk::launch_stage(2, b->phys, fp64, b->shape, b->dtab, n, b->ntiles, b->stream), "batch stage 2");
-    HIPCHK(nzk::launch_finalize(b->dtab, n, b->stream), "batch finalize");
+    if (closeEach || k == steps - 1) HIPCHK(nzk::launch_close(b->dtab, n, b->stream), "batch close");
   }
   return 0;
 }

@@ -28,11 +28,22 @@ void vfree(void*& p) {
 }
 }  // namespace
 
+// the most tiles any tile shape of this precision cuts the grid into: the capacity of the
+// per-tile partials, which carry the wave-speed max from one step to the next and so stay
+// in place when the tile shape changes
+size_t part_capacity(int Nx, int Nr, int prec) {
+  size_t most = 1;
+  for (int k = 0; k < nzk::tile_shape_count(prec == 64); ++k) {
+    int TX = 0, TY = 0;
+    nzk::tile_shape(prec == 64, k, &TX, &TY);
+    most = std::max(most, (size_t)((Nx + TX - 1) / TX) * ((Nr + TY - 1) / TY));
+  }
+  return most;
+}
+
 size_t inst_bytes(int Nx, int Nr, int prec, int NP) {
-  int TX = 0, TY = 0;
-  nzk::tile_shape(prec == 64, 0, &TX, &TY);
   const size_t N = (size_t)Nx * Nr, Ns = (N + 63) / 64 * 64, R = prec == 64 ? 8 : 4;
-  const size_t tiles = (size_t)((Nx + TX - 1) / TX) * ((Nr + TY - 1) / TY);
+  const size_t tiles = part_capacity(Nx, Nr, prec);
   return Ns * 8 * R * 2                                   // U0, U1
          + (size_t)Nr * (Nx + 1) * 4 + (size_t)(Nr + 1) * Nx * 4  // ax, ar
          + N * (16 + 4 + 4)                               // gg, code, lt
@@ -66,7 +77,9 @@ int inst_alloc(Inst& I, int Nx, int Nr, int prec, InstState* st, int shape) {
   HIPCHK(hipMalloc(&I.jw, (size_t)Nx * 16), "hipMalloc jWall");
   HIPCHK(hipMalloc(&I.src, (size_t)I.Ns * 16), "hipMalloc src");
   HIPCHK(hipMalloc(&I.tab, 256 * 3 * 16), "hipMalloc tab");
-  HIPCHK(hipMalloc(&I.part, tiles * sizeof(double2)), "hipMalloc partials");
+  I.partCap = part_capacity(Nx, Nr, prec);
+  HIPCHK(hipMalloc(&I.part, I.partCap * sizeof(double2)), "hipMalloc partials");
+  HIPCHK(hipMemset(I.part, 0, I.partCap * sizeof(double2)), "memset");
   HIPCHK(hipMalloc(&I.exStart, (tiles + 1) * sizeof(int)), "hipMalloc extra starts");
   HIPCHK(hipMalloc(&I.fcStart, (tiles + 1) * sizeof(int)), "hipMalloc far starts");
   HIPCHK(hipMalloc(&I.exFaces, sizeof(uint2)), "hipMalloc extra faces");
@@ -309,13 +322,11 @@ int inst_set_shape(Inst& I, int shape) {
   I.tilesX = (I.Nx + TX - 1) / TX;
   I.tilesY = (I.Nr + TY - 1) / TY;
   const size_t tiles = (size_t)I.tilesX * I.tilesY;
-  hfree(I.part);
+  // the partials (capacity for every shape) keep the last step's values
   hfree(I.exStart);
   hfree(I.fcStart);
-  HIPCHK(hipMalloc(&I.part, tiles * sizeof(double2)), "hipMalloc partials");
   HIPCHK(hipMalloc(&I.exStart, (tiles + 1) * sizeof(int)), "hipMalloc extra starts");
   HIPCHK(hipMalloc(&I.fcStart, (tiles + 1) * sizeof(int)), "hipMalloc far starts");
-  HIPCHK(hipMemset(I.part, 0, tiles * sizeof(double2)), "memset");
   if (int rc = build_extra_faces(I, I.codeH)) return rc;
   HIPCHK(hipMemcpy(I.code, I.codeH.data(), I.codeH.size() * 4, hipMemcpyHostToDevice), "upload code");
   return 0;

@@ -25,40 +25,27 @@
 #include <algorithm>
 
 namespace nzk {
-// Closes the step of every instance (one workgroup per instance): wave-speed max
-// and residual over its tiles, step info of the completed step (FS_STEPINFO's
-// input for the next one), spray ring head and seed (spray.js:295-296).
-__global__ void __launch_bounds__(256) k_finalize(const DevInst* __restrict__ insts, int ninst) {
-  __shared__ double red[16], redd[16];
+// Folds every instance's tile partials of the last stage 2 (one workgroup per instance): the
+// wave-speed max for the next time step, the residual of the last step and the stop
+// tests of batched solves. It runs after every step when an instance has more partials
+// than the next stage 1 folds itself, and once after each run of steps, so that what the
+// host reads is current. It changes nothing the next stage 1 would not compute alike.
+__global__ void __launch_bounds__(256) k_close(const DevInst* __restrict__ insts, int ninst) {
+  __shared__ double W[4], R[4];
   const int n = blockIdx.x;
   if (n >= ninst) return;
   const DevInst& J = insts[n];
   InstState* s = J.st;
   if (s->flags & 1u) return;
-  const int nt = J.tilesX * J.tilesY;
-  double w = 0.0, r = 0.0;
-  for (int k = threadIdx.x; k < nt; k += 256) {
-    const double2 p = J.part[k];
-    w = fmax(w, p.x);
-    r += p.y;
-  }
-  const double wb = block_max<256>(w, red);
-  const double rb = block_sum<256>(r, redd);
+  fold_groups<256>(J.part, s->nPart, threadIdx.x, W, R);
+  __syncthreads();
+  double w, r;
+  fold_combine(W, R, w, r);
   if (threadIdx.x == 0) {
-    s->last[0] = s->cur[0]; s->last[1] = s->cur[1]; s->last[2] = s->cur[2]; s->last[3] = s->cur[3];
-    s->wsnap = (unsigned long long)__double_as_longlong(wb);
-    s->wacc = 0ull;
-    s->steps += 1;
-    s->resLast = sqrt(rb / fmax(s->nGroups, 1.0)) * (double)J.c.resScale;
-    s->resAcc = 0.0;
-    if (J.c.spray) {
-      s->head = (s->head + 4u) % (uint32_t)J.c.NP;
-      s->seed = s->seed + 1u;
-    }
-    // batched solves stop on simulated time or on the residual
-    if (J.c.tEnd > 0.f && s->last[2] >= (double)J.c.tEnd) s->flags |= 1u | 4u;
-    if (J.c.resTol > 0.f && s->steps >= J.c.minSteps && s->resLast <= (double)J.c.resTol) s->flags |= 1u | 2u;
-    if (J.c.maxSteps > 0 && s->steps >= J.c.maxSteps) s->flags |= 1u | 8u;
+    s->wsnap = (unsigned long long)__double_as_longlong(w);
+    s->resAcc = r;
+    s->resLast = sqrt(r / fmax(s->nGroups, 1.0)) * (double)J.c.resScale;
+    s->flags |= stop_bits(J.c, *s, s->resLast);
   }
 }
 
@@ -221,6 +208,7 @@ __global__ void __launch_bounds__(256) k_cfl_reduce(DevInst I, int nblk) {
   if (threadIdx.x == 0) {
     I.st->wsnap = (unsigned long long)__double_as_longlong(wb);
     I.st->wacc = 0ull;
+    I.st->nPart = (uint32_t)nblk;  // the next stage 1 folds these
   }
 }
 
@@ -488,8 +476,8 @@ hipError_t launch_stage(int stage, bool phys, bool fp64, int shape, const DevIns
   const ShapeEntry& e = fp64 ? kShapes64[shape] : kShapes32[shape];
   return e.launch(stage, phys, d, ninst, ntiles, s, e0, e1);
 }
-hipError_t launch_finalize(const DevInst* d, int ninst, hipStream_t s) {
-  hipLaunchKernelGGL(k_finalize, dim3(ninst), dim3(256), 0, s, d, ninst);
+hipError_t launch_close(const DevInst* d, int ninst, hipStream_t s) {
+  hipLaunchKernelGGL(k_close, dim3(ninst), dim3(256), 0, s, d, ninst);
   return hipGetLastError();
 }
 hipError_t launch_spray(bool fp64, const DevInst* d, int ninst, int nblk, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
@@ -503,9 +491,10 @@ hipError_t launch_spray(bool fp64, const DevInst* d, int ninst, int nblk, hipStr
   return hipGetLastError();
 }
 hipError_t launch_cfl_init(bool fp64, const DevInst& I, hipStream_t s) {
-  // as many blocks as the instance has tiles (its partials array holds one per tile)
+  // one partial per block: at most ceil(N / 256), which no shape's tile count undercuts
+  // (the smallest tile holds 128 cells in FP32, 64 in FP64), so they fit the partials array
   const long long N = (long long)I.Nx * I.Nr;
-  const int nblk = (int)std::min<long long>(std::min<long long>((N + 255) / 256, 1024), (long long)I.tilesX * I.tilesY);
+  const int nblk = (int)std::min<long long>((N + 255) / 256, 1024);
   if (fp64) hipLaunchKernelGGL((k_cfl_partial<double>), dim3(nblk), dim3(256), 0, s, I);
   else hipLaunchKernelGGL((k_cfl_partial<float>), dim3(nblk), dim3(256), 0, s, I);
   hipLaunchKernelGGL(k_cfl_reduce, dim3(1), dim3(256), 0, s, I, nblk);

@@ -74,8 +74,13 @@ struct InstState {
   double resLast;  // RMS of the last completed step
   double nGroups;  // number of fluid groups (residual normalisation)
   uint32_t flags;  // bit 0: instance inactive (skipped); stopped on bit 1: the residual, 2: time, 3: step count
-  uint32_t pad;
+  uint32_t nPart;  // tile partials (wave speed, residual) the last stage-2 launch wrote (or the CFL set-up)
 };
+
+// The next step's stage 1 folds the tile partials of the last stage 2 into the time step
+// itself (one wave per workgroup reads them) when there are at most this many; with more,
+// a close kernel folds them after every step.
+constexpr uint32_t FUSED_CLOSE_MAX = 2048;
 
 struct DevInst {
   int Nx, Nr;

@@ -41,7 +41,8 @@ struct Inst {
   void* exFlux = nullptr;
   nzk::InstState* st = nullptr;  // owned unless placed in a batch's status array
   bool ownSt = true;
-  double2* part = nullptr;
+  double2* part = nullptr;       // per-tile partials of the last stage 2 (capacity partCap, every shape)
+  size_t partCap = 0;
   float* wallrow = nullptr;
   double* dint = nullptr;
   std::vector<uint8_t> mask;
@@ -58,7 +59,7 @@ struct Inst {
 // device bytes an instance of this size needs (admission by HBM budget)
 size_t inst_bytes(int Nx, int Nr, int prec, int NP);
 int inst_alloc(Inst& I, int Nx, int Nr, int prec, nzk::InstState* st, int shape = 0);
-// switch the tile shape: per-tile lists and partials are rebuilt for it
+// switch the tile shape: the per-tile lists are rebuilt for it
 int inst_set_shape(Inst& I, int shape);
 void inst_free(Inst& I);
 int inst_set_geometry(Inst& I, const uint8_t* mask, const float* geoA, const float* geoG, const float* geoM,

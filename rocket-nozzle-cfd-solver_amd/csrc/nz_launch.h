@@ -14,7 +14,7 @@ hipError_t launch_set_inst(DevInst* dst, const DevInst& src, hipStream_t s);
 // with events e0/e1, the kernel's own start and end are recorded into them
 hipError_t launch_stage(int stage, bool phys, bool fp64, int shape, const DevInst* d, int ninst, int ntiles,
                         hipStream_t s, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
-hipError_t launch_finalize(const DevInst* d, int ninst, hipStream_t s);
+hipError_t launch_close(const DevInst* d, int ninst, hipStream_t s);
 hipError_t launch_spray(bool fp64, const DevInst* d, int ninst, int nblk, hipStream_t s, hipEvent_t e0 = nullptr,
                         hipEvent_t e1 = nullptr);
 hipError_t launch_cfl_init(bool fp64, const DevInst& I, hipStream_t s);

@@ -47,8 +47,8 @@ __device__ __forceinline__ int xcd_tile(int b, int nblk, int chunk) {
 // FS_STEPINFO (solver.js:404-409): dt from the wave-speed max of the current state,
 // in the arithmetic type of the state
 template <typename R>
-__device__ __forceinline__ void step_info(const Consts& c, const InstState& s, R* cur) {
-  const R w = fmax(R(c.wFloor), R(__longlong_as_double((long long)s.wsnap)));
+__device__ __forceinline__ void step_info(const Consts& c, const InstState& s, double wmax, R* cur) {
+  const R w = fmax(R(c.wFloor), R(wmax));
   R dt = fmin(R(c.cfl) / w, R(c.dtMax));
   const R lastDt = R(s.last[0]), lastT = R(s.last[2]);
   if (lastDt > R(0)) dt = fmin(dt, lastDt * R(1.2f));
@@ -56,6 +56,55 @@ __device__ __forceinline__ void step_info(const Consts& c, const InstState& s, R
   cur[1] = lastT;
   cur[2] = lastT + dt;
   cur[3] = w;
+}
+
+// Wave-speed max and residual sum of an instance's tile partials, in one fixed order for
+// every caller: 256 virtual lanes v take partials v, v + 256, ... in turn; each group of
+// 64 lanes folds by a butterfly (which leaves the same bits in all its lanes); the four
+// group results combine as max and (R0 + R1) + (R2 + R3). A workgroup of NT threads (NT
+// divides 256) runs virtual lanes tid, tid + NT, ...; the loads of four consecutive
+// partials of a lane are issued together. fold_groups leaves the group results in W[4],
+// R[4] (LDS), which fold_combine reads after a barrier.
+template <int NT>
+__device__ __forceinline__ void fold_groups(const double2* __restrict__ part, uint32_t n, int tid, double* W, double* R) {
+  static_assert(256 % NT == 0 && NT % 64 == 0, "fold: the workgroup size must divide 256");
+#pragma unroll
+  for (int m = 0; m < 256 / NT; ++m) {
+    const uint32_t v = (uint32_t)(tid + m * NT);
+    double w = 0.0, r = 0.0;
+    for (uint32_t k0 = v; k0 < n; k0 += 1024) {
+      double2 p[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const uint32_t k = k0 + 256u * j;
+        p[j] = k < n ? part[k] : make_double2(0.0, 0.0);
+      }
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        w = fmax(w, p[j].x);
+        r += p[j].y;
+      }
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+      w = fmax(w, __shfl_xor(w, o, 64));
+      r += __shfl_xor(r, o, 64);
+    }
+    if ((v & 63u) == 0u) { W[v >> 6] = w; R[v >> 6] = r; }
+  }
+}
+__device__ __forceinline__ void fold_combine(const double* W, const double* R, double& w, double& r) {
+  w = fmax(fmax(W[0], W[1]), fmax(W[2], W[3]));
+  r = (R[0] + R[1]) + (R[2] + R[3]);
+}
+
+// Stop tests of batched solves after the last completed step: simulated time, residual
+// (after a minimum step count) and step count; 0 while the solve goes on
+__device__ __forceinline__ uint32_t stop_bits(const Consts& c, const InstState& s, double resLast) {
+  uint32_t f = 0u;
+  if (c.tEnd > 0.f && s.last[2] >= (double)c.tEnd) f |= 1u | 4u;
+  if (c.resTol > 0.f && s.steps > 0 && s.steps >= c.minSteps && resLast <= (double)c.resTol) f |= 1u | 2u;
+  if (c.maxSteps > 0 && s.steps >= c.maxSteps) f |= 1u | 8u;
+  return f;
 }
 
 template <int NT>
@@ -254,6 +303,7 @@ struct StageSmem {
   R PF[8][MAXF];
   unsigned char fmap[WH][WW];                      // far-cell slot by window position
   int nEx, nF;
+  double foldW[4], foldR[4];                       // group results of the last step's partials (stage 1)
   double red[TX * TY / 64];                        // one slot per wave of the block reductions
   double redd[TX * TY / 64];
 };
@@ -271,7 +321,6 @@ __global__ void __launch_bounds__(TX* TY, (STAGE == 2 && PHYS) ? 1 : W) k_stage(
   const DevInst& I = insts[ii];
   const Consts& c = I.c;
   InstState* st = I.st;
-  const bool active = !(st->flags & 1u);
   const int Nx = I.Nx, Nr = I.Nr;
   const long long Ns = I.Ns;
   const int lt_ = t - I.tile0;
@@ -280,16 +329,58 @@ __global__ void __launch_bounds__(TX* TY, (STAGE == 2 && PHYS) ? 1 : W) k_stage(
   R* __restrict__ U1 = static_cast<R*>(I.U1);
   const R* __restrict__ Uin = STAGE == 1 ? U0 : U1;
 
-  // time step of this step: FS_STEPINFO at the start of the step
-  R cur[4];
+  // Step state. Stage 1 closes the last step and opens this one: the stop tests of the
+  // last completed step (simulated time and step count here; with the residual once the
+  // partials are folded), then FS_STEPINFO (solver.js:404-409) from the wave-speed max
+  // of the partials the last stage 2 wrote, folded here by wave 0 of every workgroup
+  // (`fused`) or by the close kernel that followed it. Stage 2 takes the time step stage
+  // 1 wrote; its first workgroup files the step as completed. Each field is written in a
+  // launch in which no workgroup reads it.
+  const bool was = !(st->flags & 1u);
+  bool active = was;
+  R cur[4] = {R(0), R(0), R(0), R(0)};
+  R dt = R(0), tsel = R(0);
+  uint32_t nPart = 0u;
+  bool fused = false;
   if (STAGE == 1) {
-    step_info<R>(c, *st, cur);
-    if (lt_ == 0 && tid == 0 && active) { st->cur[0] = cur[0]; st->cur[1] = cur[1]; st->cur[2] = cur[2]; st->cur[3] = cur[3]; }
+    nPart = st->nPart;
+    fused = nPart <= FUSED_CLOSE_MAX;
+    tsel = R(st->last[2]);
+    if (was) {
+      const uint32_t f = stop_bits(c, *st, INFINITY);
+      if (f) {
+        active = false;
+        if (lt_ == 0) {
+          // the solve stops here; its last step's residual still decides the stop reason
+          if (fused) fold_groups<NT>(I.part, nPart, tid, sm.foldW, sm.foldR);
+          __syncthreads();
+          if (tid == 0) {
+            if (fused) {
+              double w, r;
+              fold_combine(sm.foldW, sm.foldR, w, r);
+              st->wsnap = (unsigned long long)__double_as_longlong(w);
+              st->resAcc = r;
+              st->resLast = sqrt(r / fmax(st->nGroups, 1.0)) * (double)c.resScale;
+            }
+            st->flags |= stop_bits(c, *st, st->resLast);
+          }
+        }
+      }
+    }
   } else {
     cur[0] = R(st->cur[0]); cur[1] = R(st->cur[1]); cur[2] = R(st->cur[2]); cur[3] = R(st->cur[3]);
+    dt = cur[0];
+    tsel = cur[2];
+    if (active && lt_ == 0 && tid == 0) {
+      st->last[0] = st->cur[0]; st->last[1] = st->cur[1]; st->last[2] = st->cur[2]; st->last[3] = st->cur[3];
+      st->steps += 1;
+      if (c.spray) {
+        st->head = (st->head + 4u) % (uint32_t)c.NP;
+        st->seed = st->seed + 1u;
+      }
+      st->nPart = (uint32_t)(I.tilesX * I.tilesY);
+    }
   }
-  const R dt = cur[0];
-  const R tsel = STAGE == 1 ? cur[1] : cur[2];
 
   double wloc = 0.0;
   double rloc = 0.0;
@@ -340,6 +431,8 @@ __global__ void __launch_bounds__(TX* TY, (STAGE == 2 && PHYS) ? 1 : W) k_stage(
         if (dirR ? (fi < Nx && fj <= Nr) : (fi <= Nx && fj < Nr)) area[r] = face_area(I, dirR, fi, fj);
       }
     }
+    // the last step's partials are folded while the tile's loads are in flight
+    if (STAGE == 1 && fused) fold_groups<NT>(I.part, nPart, tid, sm.foldW, sm.foldR);
 #pragma unroll
     for (int r = 0; r < CPT; ++r) {
       const int idx = tid + r * NT;
@@ -411,6 +504,25 @@ __global__ void __launch_bounds__(TX* TY, (STAGE == 2 && PHYS) ? 1 : W) k_stage(
       return prim_at<R>(I, Uin, gi, gj, tsel);
     };
     __syncthreads();
+
+    // this step's time step (stage 1); the first workgroup records it, with the folded
+    // wave speed and residual of the last step and that step's residual stop test
+    if (STAGE == 1) {
+      double wF = 0.0, rF = 0.0;
+      if (fused) fold_combine(sm.foldW, sm.foldR, wF, rF);
+      const double wmax = fused ? wF : __longlong_as_double((long long)st->wsnap);
+      step_info<R>(c, *st, wmax, cur);
+      dt = cur[0];
+      if (lt_ == 0 && tid == 0) {
+        st->cur[0] = cur[0]; st->cur[1] = cur[1]; st->cur[2] = cur[2]; st->cur[3] = cur[3];
+        if (fused) {
+          st->wsnap = (unsigned long long)__double_as_longlong(wF);
+          st->resAcc = rF;
+          st->resLast = sqrt(rF / fmax(st->nGroups, 1.0)) * (double)c.resScale;
+          st->flags |= stop_bits(c, *st, st->resLast);
+        }
+      }
+    }
 
     // ---- 2. each face flux once (FS_FLUX_X / FS_FLUX_R), times |area|: the west and
     // the south face of every tile cell (one each per thread, direction fixed at compile
@@ -815,7 +927,8 @@ __global__ void __launch_bounds__(TX* TY, (STAGE == 2 && PHYS) ? 1 : W) k_stage(
   }
 
   if (STAGE == 2) {
-    // per-tile partials, reduced by k_finalize (no same-address atomics across workgroups)
+    // per-tile partials, folded by the next stage 1 or the close kernel (no same-address
+    // atomics across workgroups)
     const double wb = block_max<NT>(wloc, sm.red);
     const double rb = block_sum<NT>(rloc, sm.redd);
     if (tid == 0) I.part[lt_] = make_double2(active ? wb : 0.0, active ? rb : 0.0);

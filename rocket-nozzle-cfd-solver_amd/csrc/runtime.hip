@@ -213,7 +213,8 @@ static int enqueue_step(nz_solver* s, bool phys, bool spray, hipEvent_t* ev) {
            "spray");
   HIPCHK(nzk::launch_stage(2, phys, fp64, s->I.shape, s->dinst, 1, ntiles, s->stream, ev ? ev[4] : nullptr,
                            ev ? ev[5] : nullptr), "stage 2");
-  HIPCHK(nzk::launch_finalize(s->dinst, 1, s->stream), "finalize");
+  // more partials than the next stage 1 folds itself: fold them now
+  if ((unsigned)ntiles > nzk::FUSED_CLOSE_MAX) HIPCHK(nzk::launch_close(s->dinst, 1, s->stream), "close");
   return 0;
 }
 
@@ -227,6 +228,8 @@ int nz_step(nz_solver* s, int n, const nz_params* P) {
   const bool spray = s->h.c.spray != 0;
   for (int k = 0; k < n; ++k)
     if (int rc = enqueue_step(s, phys, spray, nullptr)) return rc;
+  // fold the last step's partials, so that the step state the host reads is current
+  if (n > 0) HIPCHK(nzk::launch_close(s->dinst, 1, s->stream), "close");
   s->stepCount += n;
   return 0;
 }
@@ -247,6 +250,7 @@ int nz_step_timed(nz_solver* s, int n, const nz_params* P, double* ms) {
   if (hipEventRecord(span[0], s->stream) != hipSuccess) rc = nz_fail(-10, "event");
   for (int k = 0; k < n && !rc; ++k) rc = enqueue_step(s, phys, spray, &ev[(size_t)k * 6]);
   if (!rc && hipEventRecord(span[1], s->stream) != hipSuccess) rc = nz_fail(-10, "event");
+  if (!rc && nzk::launch_close(s->dinst, 1, s->stream) != hipSuccess) rc = nz_fail(-10, "close");
   HIPCHK(hipStreamSynchronize(s->stream), "nz_step_timed sync");
   double t[3] = {0, 0, 0};
   float all = 0.f;
@@ -301,12 +305,14 @@ struct Snapshot {
   void* U1 = nullptr;
   float* parc = nullptr;
   float* src = nullptr;
+  void* part = nullptr;  // the last step's tile partials, folded by the next step
   InstState st{};
   ~Snapshot() {
     if (U0) (void)hipFree(U0);
     if (U1) (void)hipFree(U1);
     if (parc) (void)hipFree(parc);
     if (src) (void)hipFree(src);
+    if (part) (void)hipFree(part);
   }
 };
 int snap_take(nz_solver* s, Snapshot& z) {
@@ -320,6 +326,8 @@ int snap_take(nz_solver* s, Snapshot& z) {
   HIPCHK(hipMemcpyAsync(z.U1, I.U1, us, hipMemcpyDeviceToDevice, s->stream), "snapshot");
   HIPCHK(hipMemcpyAsync(z.parc, I.parc, ps, hipMemcpyDeviceToDevice, s->stream), "snapshot");
   HIPCHK(hipMemcpyAsync(z.src, I.src, ss, hipMemcpyDeviceToDevice, s->stream), "snapshot");
+  HIPCHK(hipMalloc(&z.part, I.partCap * sizeof(double2)), "snapshot");
+  HIPCHK(hipMemcpyAsync(z.part, I.part, I.partCap * sizeof(double2), hipMemcpyDeviceToDevice, s->stream), "snapshot");
   HIPCHK(hipMemcpyAsync(&z.st, I.st, sizeof(InstState), hipMemcpyDeviceToHost, s->stream), "snapshot");
   HIPCHK(hipStreamSynchronize(s->stream), "snapshot");
   return 0;
@@ -331,6 +339,7 @@ int snap_restore(nz_solver* s, const Snapshot& z) {
   HIPCHK(hipMemcpyAsync(I.U1, z.U1, us, hipMemcpyDeviceToDevice, s->stream), "restore");
   HIPCHK(hipMemcpyAsync(I.parc, z.parc, ps, hipMemcpyDeviceToDevice, s->stream), "restore");
   HIPCHK(hipMemcpyAsync(I.src, z.src, ss, hipMemcpyDeviceToDevice, s->stream), "restore");
+  HIPCHK(hipMemcpyAsync(I.part, z.part, I.partCap * sizeof(double2), hipMemcpyDeviceToDevice, s->stream), "restore");
   HIPCHK(hipMemcpyAsync(I.st, &z.st, sizeof(InstState), hipMemcpyHostToDevice, s->stream), "restore");
   HIPCHK(hipStreamSynchronize(s->stream), "restore");
   return 0;
