@@ -3,10 +3,11 @@
 // Resident solves of any grid size share every launch: the stage kernels run one
 // workgroup per tile over the concatenated tile ranges of all residents (each
 // workgroup finds its instance by binary search over the descriptor table), the
-// spray kernel over their concatenated parcel blocks, the finalize kernel one
-// workgroup per resident. Per-instance stop tests (simulated time, residual, step
-// count) run on the device in the finalize kernel; a stopped instance's workgroups
-// return at once, so nothing waits on the host between steps. Between run calls the
+// spray kernel over their concatenated parcel blocks, the close kernel one workgroup
+// per resident. Per-instance stop tests (simulated time, residual, step count) run on
+// the device at the start of each step's stage 1 and in the close kernel after the run;
+// a stopped instance's workgroups return at once, so nothing waits on the host between
+// steps. Between run calls the
 // host retires stopped solves (freeing their buffers) and admits queued jobs
 // first-fit by device-memory budget. The step sequence of a run is captured in a
 // HIP graph and replayed while the resident set and step count stay the same.
