@@ -119,6 +119,22 @@ __device__ __forceinline__ double block_max(double v, double* red) {
   }
   return m;  // valid on thread 0
 }
+// max of w and sum of r over the workgroup in one pass (one barrier); valid on thread 0
+template <int NT>
+__device__ __forceinline__ void block_max_sum(double& w, double& r, double* redW, double* redR) {
+  for (int o = 32; o > 0; o >>= 1) {
+    w = fmax(w, __shfl_xor(w, o, 64));
+    r += __shfl_xor(r, o, 64);
+  }
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if (lane == 0) { redW[wv] = w; redR[wv] = r; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    w = 0.0;
+    r = 0.0;
+    for (int k = 0; k < NT / 64; ++k) { w = fmax(w, redW[k]); r += redR[k]; }
+  }
+}
 template <int NT>
 __device__ __forceinline__ double block_sum(double v, double* red) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
@@ -929,8 +945,8 @@ __global__ void __launch_bounds__(TX* TY, (STAGE == 2 && PHYS) ? 1 : W) k_stage(
   if (STAGE == 2) {
     // per-tile partials, folded by the next stage 1 or the close kernel (no same-address
     // atomics across workgroups)
-    const double wb = block_max<NT>(wloc, sm.red);
-    const double rb = block_sum<NT>(rloc, sm.redd);
+    double wb = wloc, rb = rloc;
+    block_max_sum<NT>(wb, rb, sm.red, sm.redd);
     if (tid == 0) I.part[lt_] = make_double2(active ? wb : 0.0, active ? rb : 0.0);
   }
 }
