@@ -74,8 +74,9 @@ __global__ void __launch_bounds__(256) k_spray(const DevInst* __restrict__ insts
   float* __restrict__ pc = I.parc;
   const R dt = R(st->cur[0]);
   const uint32_t head = st->head, seed = st->seed;
-  const R ax_ = pc[idx], ay = pc[NP + idx], az = pc[2 * NP + idx], aw = pc[3 * NP + idx];
-  const R bx = pc[4 * NP + idx], by = pc[5 * NP + idx], bz = pc[6 * NP + idx], bw = pc[7 * NP + idx];
+  const unsigned ui = (unsigned)idx, un = (unsigned)NP;
+  const R ax_ = gld(pc, ui), ay = gld(pc, un + ui), az = gld(pc, 2 * un + ui), aw = gld(pc, 3 * un + ui);
+  const R bx = gld(pc, 4 * un + ui), by = gld(pc, 5 * un + ui), bz = gld(pc, 6 * un + ui), bw = gld(pc, 7 * un + ui);
   int rel = idx - (int)head;
   if (rel < 0) rel += NP;
   R o[8] = {ax_, ay, az, aw, bx, by, bz, bw};
@@ -90,14 +91,14 @@ __global__ void __launch_bounds__(256) k_spray(const DevInst* __restrict__ insts
     const float* e3 = I.tab + (512 + tt) * 4;
     const R ang = R(e1[2]) + (R(rnd(id, 2u)) - R(0.5)) * R(2) * R(e2[2]);
     const R r = fmax(R(e1[1]) + (R(rnd(id, 3u)) - R(0.5)) * R(e2[3]), R(0.05f));
-    const R d = R(e2[1]) * exp((R(rnd(id, 4u)) - R(0.5)) * R(1.4f));
-    o[0] = e1[0]; o[1] = r; o[2] = R(e1[3]) * cos(ang); o[3] = R(e1[3]) * sin(ang);
+    const R d = R(e2[1]) * expR((R(rnd(id, 4u)) - R(0.5)) * R(1.4f));
+    o[0] = e1[0]; o[1] = r; o[2] = R(e1[3]) * cosR(ang); o[3] = R(e1[3]) * sinR(ang);
     o[4] = d; o[5] = R(c.sprayMhat) * dt / R(perStep); o[6] = e2[0]; o[7] = e3[0];
     if (by > R(0)) { dmo = by; dpx = by * az; dpy = by * aw; }
   } else if (by > R(0)) {
     const int gi = min(max((int)floor(ax_), 0), Nx - 1), gj = min(max((int)floor(ay), 0), Nr - 1);
     const long long kg = (long long)gj * Nx + gi;
-    if (I.code[kg] & C_SOLID) {
+    if (gld(I.code, (unsigned)kg) & C_SOLID) {
       // born inside solid hardware: walk out into the flow
       o[0] = ax_ + R(1); o[1] = fmax(ay - R(0.5), R(0.05f)); o[2] = fmax(az, R(0)); o[3] = aw;
     } else {
@@ -109,36 +110,36 @@ __global__ void __launch_bounds__(256) k_spray(const DevInst* __restrict__ insts
       const Prim<R> w = load_prim<R>(U1, Ns, kg);
       {
         const R rm = (R(1) - w.Z) * R(c.gas[0]) + w.F * R(c.Rs[0]) + w.O * R(c.Rs[1]) + fmax(w.Z - w.F - w.O, R(0)) * R(c.Rs[2]);
-        const R T = w.p / (w.r * rm) * R(c.gas[2]);
+        const R T = dv(w.p, w.r * rm) * R(c.gas[2]);
         const R rhoG = w.r * R(1.225f);
         const R rvx = w.u - vx, rvy = w.v - vy;
-        const R Urel = sqrt(rvx * rvx + rvy * rvy) * R(287.6f) + R(1e-3f);
-        const R mu = R(1.458e-6f) * T * sqrt(T) / (T + R(110.4f));
-        const R Re = rhoG * Urel * d / mu;
-        const R invTau = R(18) * mu * (R(1) + R(0.15f) * pow(fmin(Re, R(1000)), R(0.687f))) / (pr0 * d * d);
-        const R ex = exp(-dts * invTau);
+        const R Urel = sq(rvx * rvx + rvy * rvy) * R(287.6f) + R(1e-3f);
+        const R mu = dv(R(1.458e-6f) * T * sq(T), T + R(110.4f));
+        const R Re = dv(rhoG * Urel * d, mu);
+        const R invTau = dv(R(18) * mu * (R(1) + R(0.15f) * powR(fmin(Re, R(1000)), R(0.687f))), pr0 * d * d);
+        const R ex = expR(-dts * invTau);
         vx = w.u + (vx - w.u) * ex;
         vy = w.v + (vy - w.v) * ex;
         const R cp = rm * R(287.05f) * (w.G + R(1));
-        const R Bn = fmax(cp * (T - pr1), R(0)) / pr2;
-        const R K = R(8) * (mu / R(0.7f)) / pr0 * log(R(1) + Bn) * (R(1) + R(0.27f) * sqrt(Re));
+        const R Bn = dv(fmax(cp * (T - pr1), R(0)), pr2);
+        const R K = dv(R(8) * dv(mu, R(0.7f)), pr0) * logR(R(1) + Bn) * (R(1) + R(0.27f) * sq(Re));
         const R d2 = d * d - K * dts;
-        dn = d2 > R(1e-12f) ? sqrt(d2) : R(0);
-        const R f = dn / d;
+        dn = d2 > R(1e-12f) ? sq(d2) : R(0);
+        const R f = dv(dn, d);
         mn = dn > R(0) ? m * f * f * f : R(0);
-        const R we = rhoG * Urel * Urel * dn / pr3;
+        const R we = dv(rhoG * Urel * Urel * dn, pr3);
         if (we > R(12) && dn > R(0)) {
-          const R dst = R(12) * pr3 / (rhoG * Urel * Urel);
-          const R tb = R(3) * dn / Urel * sqrt(pr0 / rhoG);
-          dn = mixR(dn, fmax(dst, R(2e-6f)), R(1) - exp(-dts / tb));
+          const R dst = dv(R(12) * pr3, rhoG * Urel * Urel);
+          const R tb = dv(R(3) * dn, Urel) * sq(dv(pr0, rhoG));
+          dn = mixR(dn, fmax(dst, R(2e-6f)), R(1) - expR(-dv(dts, tb)));
         }
       }
       R xi = bw;
       if (xi > R(0) && ax_ >= xi) {
         const uint32_t id = (uint32_t)idx ^ (seed * 2654435761u);
-        const R sp = sqrt(vx * vx + vy * vy), phi = (R(rnd(id, 5u)) - R(0.5)) * R(2) * R(c.fan);
-        vx = sp * cos(phi);
-        vy = sp * sin(phi);
+        const R sp = sq(vx * vx + vy * vy), phi = (R(rnd(id, 5u)) - R(0.5)) * R(2) * R(c.fan);
+        vx = sp * cosR(phi);
+        vy = sp * sinR(phi);
         dn *= R(0.25);
         xi = R(0);
       }
@@ -167,7 +168,7 @@ __global__ void __launch_bounds__(256) k_spray(const DevInst* __restrict__ insts
       }
     }
   }
-  for (int q = 0; q < 8; ++q) pc[q * NP + idx] = (float)o[q];
+  for (int q = 0; q < 8; ++q) gst(pc, (unsigned)q * un + ui, (float)o[q]);
   // deposit into the host of the cell the parcel started in
   if (!(ax_ < R(0)) && !(dmo == R(0) && dpx == R(0) && dpy == R(0))) {
     int gi = min(max((int)floor(ax_), 0), Nx - 1), gj = min(max((int)floor(ay), 0), Nr - 1);

@@ -159,6 +159,14 @@ __device__ __forceinline__ float powR(float x, float y) { return __builtin_amdgc
 __device__ __forceinline__ double powR(double x, double y) { return pow(x, y); }
 __device__ __forceinline__ float expR(float x) { return __builtin_amdgcn_exp2f(x * 1.44269504088896341f); }
 __device__ __forceinline__ double expR(double x) { return exp(x); }
+// natural logarithm, sine and cosine likewise (GLSL's log, sin, cos are implementation
+// precision; the hardware v_log_f32 / v_sin_f32 / v_cos_f32 serve FP32)
+__device__ __forceinline__ float logR(float x) { return __builtin_amdgcn_logf(x) * 0.693147180559945309f; }
+__device__ __forceinline__ double logR(double x) { return log(x); }
+__device__ __forceinline__ float sinR(float x) { return __sinf(x); }
+__device__ __forceinline__ double sinR(double x) { return sin(x); }
+__device__ __forceinline__ float cosR(float x) { return __cosf(x); }
+__device__ __forceinline__ double cosR(double x) { return cos(x); }
 
 template <typename R>
 __device__ __forceinline__ R clampR(R x, R lo, R hi) { return fmin(fmax(x, lo), hi); }
