@@ -558,16 +558,34 @@ __global__ void __launch_bounds__(TX* TY, (STAGE == 2 && PHYS) ? 1 : W) k_stage(
       face_flux_w<R, 1>(I, true, i0 + tx, j0 + ty, tileP, tileA, F, FY, FO);
       putR(tx, ty, F, FY, FO);
     }
-    for (int f = tid; f < TY + TX + nEx; f += NT) {
+    // The tile's north row of r faces, its east column of x faces and the extra faces.
+    // Shapes tuned for latency (W < 4: one wave of workgroups fills the GPU) start each
+    // kind on a wave of its own (work item v: [0, TX) north faces, from B_E the east
+    // faces, from B_X the extra faces), so the kinds run side by side on different SIMDs
+    // rather than one after another in one divergent wave, each with the flux routine
+    // specialised for its direction; the throughput shapes (W = 4, many workgroups per CU
+    // in turn) pack the tile's faces into as few waves as possible and evaluate them
+    // with the one routine that takes the direction at run time.
+    constexpr bool SPLIT = W < 4;
+    constexpr int B_E = SPLIT ? (TX + 63) / 64 * 64 : TX, B_X = B_E + (SPLIT ? (TY + 63) / 64 * 64 : TY);
+    for (int v = tid; v < B_X + nEx; v += NT) {
       R F[4], FY[4], FO;
-      if (f < TY + TX) {
-        const bool dirR = f >= TY;
-        const int fx = dirR ? f - TY : TX, fy = dirR ? TY : f;
-        face_flux_w<R>(I, dirR, i0 + fx, j0 + fy, tileP, tileA, F, FY, FO);
-        if (dirR) putR(fx, fy, F, FY, FO);
-        else putX(fx, fy, F, FY, FO);
+      if (v < B_X) {
+        const bool dirR = v < B_E;
+        const int k = dirR ? v : v - B_E;
+        if (k < (dirR ? TX : TY)) {
+          const int fx = dirR ? k : TX, fy = dirR ? TY : k;
+          if constexpr (SPLIT) {
+            if (dirR) face_flux_w<R, 1>(I, true, i0 + fx, j0 + fy, tileP, tileA, F, FY, FO);
+            else face_flux_w<R, 0>(I, false, i0 + fx, j0 + fy, tileP, tileA, F, FY, FO);
+          } else {
+            face_flux_w<R>(I, dirR, i0 + fx, j0 + fy, tileP, tileA, F, FY, FO);
+          }
+          if (dirR) putR(fx, fy, F, FY, FO);
+          else putX(fx, fy, F, FY, FO);
+        }
       } else {
-        const int e = f - TY - TX;
+        const int e = v - B_X;
         const uint2 ef = sm.ex[e];
         face_flux_w<R>(I, ex_dir(ef), ex_i(ef), ex_j(ef), anyP, globA, F, FY, FO);
         for (int q = 0; q < 4; ++q) { gst(exFlux, e * 16 + q, F[q]); gst(exFlux, e * 16 + 4 + q, FY[q]); }
@@ -745,16 +763,24 @@ __global__ void __launch_bounds__(TX* TY, (STAGE == 2 && PHYS) ? 1 : W) k_stage(
           vface_w<R, 1>(I, true, i0 + tx, j0 + ty, tileP, tileT, tileA, F, FY);
           putVR(tx, ty, F, FY);
         }
-        for (int f = tid; f < TY + TX + nEx; f += NT) {
+        for (int v = tid; v < B_X + nEx; v += NT) {
           R F[4], FY[3];
-          if (f < TY + TX) {
-            const bool dirR = f >= TY;
-            const int fx = dirR ? f - TY : TX, fy = dirR ? TY : f;
-            vface_w<R>(I, dirR, i0 + fx, j0 + fy, tileP, tileT, tileA, F, FY);
-            if (dirR) putVR(fx, fy, F, FY);
-            else putVX(fx, fy, F, FY);
+          if (v < B_X) {
+            const bool dirR = v < B_E;
+            const int k = dirR ? v : v - B_E;
+            if (k < (dirR ? TX : TY)) {
+              const int fx = dirR ? k : TX, fy = dirR ? TY : k;
+              if constexpr (SPLIT) {
+                if (dirR) vface_w<R, 1>(I, true, i0 + fx, j0 + fy, tileP, tileT, tileA, F, FY);
+                else vface_w<R, 0>(I, false, i0 + fx, j0 + fy, tileP, tileT, tileA, F, FY);
+              } else {
+                vface_w<R>(I, dirR, i0 + fx, j0 + fy, tileP, tileT, tileA, F, FY);
+              }
+              if (dirR) putVR(fx, fy, F, FY);
+              else putVX(fx, fy, F, FY);
+            }
           } else {
-            const int e = f - TY - TX;
+            const int e = v - B_X;
             const uint2 ef = sm.ex[e];
             vface_w<R>(I, ex_dir(ef), ex_i(ef), ex_j(ef), anyP, anyT, globA, F, FY);
             for (int q = 0; q < 4; ++q) gst(exFlux, e * 16 + 9 + q, F[q]);
