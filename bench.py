@@ -52,25 +52,33 @@ def cpu_model():
     return platform.processor()
 
 
-def cpu_baseline(setup, budget_s=15.0):
+def cpu_baseline(setup, budget_s=15.0, budget1_s=5.0):
     """The oracle (FP32 CPU restatement, OpenMP) on the same workload for a bounded
-    number of steps; cell-updates/s on this host's cores."""
+    number of steps; cell-updates/s on this host's cores, and on one core beside it."""
     import oracle
     threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count() or 1
     threads = min(threads, 16)
-    o = oracle.Oracle(setup.Nx, setup.Nr, 32, threads=threads)
-    o.set_geometry(setup.mask, setup.geoA, setup.geoG, setup.geoM, setup.jWall)
-    o.set_injector(setup.injector)
-    A, B = setup.initial_state()
-    o.set_state(A, B)
-    o.step(2, setup.P)  # warm the pages
-    steps, t0 = 0, time.perf_counter()
-    while time.perf_counter() - t0 < budget_s:
-        o.step(5, setup.P)
-        steps += 5
-    dt = time.perf_counter() - t0
-    return {"value": setup.Nx * setup.Nr * steps / dt, "unit": "cell-updates/s", "cores": threads, "kind": "port",
-            "sample": f"oracle FP32 on {setup.Nx}x{setup.Nr}, {steps} steps in {dt:.1f} s ({cpu_model()})"}
+
+    def timed(nthreads, budget, chunk):
+        o = oracle.Oracle(setup.Nx, setup.Nr, 32, threads=nthreads)
+        o.set_geometry(setup.mask, setup.geoA, setup.geoG, setup.geoM, setup.jWall)
+        o.set_injector(setup.injector)
+        A, B = setup.initial_state()
+        o.set_state(A, B)
+        o.step(2, setup.P)  # warm the pages
+        steps, t0 = 0, time.perf_counter()
+        while time.perf_counter() - t0 < budget:
+            o.step(chunk, setup.P)
+            steps += chunk
+        return steps, time.perf_counter() - t0
+
+    steps, dt = timed(threads, budget_s, 5)
+    steps1, dt1 = timed(1, budget1_s, 1)
+    cells = setup.Nx * setup.Nr
+    return {"value": cells * steps / dt, "unit": "cell-updates/s", "cores": threads, "kind": "port",
+            "sample": f"oracle FP32 on {setup.Nx}x{setup.Nr}, {steps} steps in {dt:.1f} s ({cpu_model()}, "
+                      f"nproc {os.cpu_count()})",
+            "one_core": {"value": cells * steps1 / dt1, "sample": f"{steps1} steps in {dt1:.1f} s on 1 thread"}}
 
 
 def pct(v, q):
