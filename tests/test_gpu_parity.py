@@ -306,6 +306,92 @@ def test_integrals_match_host_sums(cold):
     assert I["mass"] > 0 and I["residual"] >= 0
 
 
+def plane_integrals(s, A, B):
+    """Exit-plane thrust and exhaust mass flow (simsetup.js:314-337), throat mass flow
+    (simsetup.js:239-250) and chamber pressure at (iChamber, 0) (app.js:537) of a state,
+    in grid units, summed in float64 on the host."""
+    A = A.reshape(s.Nr, s.Nx, 4).astype(np.float64)
+    B = B.reshape(s.Nr, s.Nx, 4).astype(np.float64)
+    Ax = s.Ax.reshape(s.Nr + 1, s.Nx + 1).astype(np.float64)
+    mask = s.mask.reshape(s.Nr, s.Nx)
+
+    def prim(j, i):
+        rho = max(A[j, i, 0], 1e-9)
+        u, v = A[j, i, 1] / rho, A[j, i, 2] / rho
+        return rho, u, max((A[j, i, 3] - 0.5 * rho * (u * u + v * v)) / B[j, i, 2], 1e-9)
+
+    F = md = mt = 0.0
+    for j in range(s.jExit):
+        rho, u, p = prim(j, s.iExit)
+        dA = 2 * np.pi * abs(Ax[j, s.iExit + 1])
+        F += (rho * u * u + p - s.P.amb[3]) * dA
+        md += B[j, s.iExit, 0] * u * dA
+    j = 0
+    while j < s.Nr and not mask[j, s.iThroat]:
+        rho, u, _ = prim(j, s.iThroat)
+        mt += rho * u * 2 * np.pi * abs(Ax[j, s.iThroat])
+        j += 1
+    return {"F": F, "mdot": md, "mdotThroat": mt, "Pc": prim(0, s.iChamber)[2]}
+
+
+def test_converged_integrals(cold):
+    """Integrated thrust, exit and throat mass flow and chamber pressure on the converged
+    cold-cone flow agree within 0.1 %:
+    - FP32 against FP64 runs of the same kernels from the initial state, 15,000 steps
+      (the flow is converged there: the FP64 integrals move by less than 0.1 % over the
+      last 1,000 steps; the exit plane carries an unsteady over-expanded plume, about
+      1e-4);
+    - the HIP solver against the oracle, both started from the converged FP64 state
+      (rounded to float32) and advanced 300 steps with the ignition ramp off (the state
+      is past it; restarting the clock would otherwise replay the ramp), integrals and
+      fields.
+    Device integrals are the FP64 sums of the solver (nz_integrals_read); the oracle's
+    state goes through the same host sums as the solver's."""
+    keys = ("F", "mdot", "mdotThroat", "Pc")
+    runs = {}
+    for prec in (64, 32):
+        g = GPUSolver(precision=prec)
+        load_setup(g, cold)
+        g.step(14000, cold.P)
+        before = g.integrals(cold.P)
+        g.step(1000, cold.P)
+        runs[prec] = (g, before, g.integrals(cold.P))
+    g64, b64, i64 = runs[64]
+    _, _, i32 = runs[32]
+    drift = {k: abs(i64[k] / b64[k] - 1) for k in keys}
+    e32 = {k: abs(i32[k] / i64[k] - 1) for k in keys}
+    say("[parity] cold cone 15000 steps: FP64 drift over the last 1000 steps " + " ".join(f"{k}={v:.1e}" for k, v in drift.items()))
+    say("[parity] cold cone 15000 steps: FP32 vs FP64 integrals " + " ".join(f"{k}={v:.1e}" for k, v in e32.items()))
+    assert max(drift.values()) < 1e-3
+    assert max(e32.values()) < 1e-3
+
+    # continuation from the converged state: HIP FP32 against the FP32 oracle
+    st = g64.readAll()
+    A, B = st["A"].astype(np.float32), st["B"].astype(np.float32)
+    P = type(cold.P).from_buffer_copy(cold.P)
+    P.ramp = 0.0
+    g = GPUSolver()
+    g.resize(cold.Nx, cold.Nr)
+    g.setGeometry({"mask": cold.mask, "geoA": cold.geoA, "geoG": cold.geoG, "geoM": cold.geoM, "jWall": cold.jWall})
+    g.setState(A, B)
+    o = orc.Oracle(cold.Nx, cold.Nr, 32)
+    o.set_geometry(cold.mask, cold.geoA, cold.geoG, cold.geoM, cold.jWall)
+    o.set_state(A, B)
+    g.step(300, P)
+    o.step(300, P)
+    G = g.readAll()
+    Ao, Bo = o.state()
+    ig, io = plane_integrals(cold, G["A"], G["B"]), plane_integrals(cold, Ao, Bo)
+    eo = {k: abs(ig[k] / io[k] - 1) for k in keys}
+    pg, po = prims(G["A"], G["B"]), prims(Ao, Bo)
+    errs = {k: rel_l2(pg[k], po[k]) for k in pg}
+    say("[parity] converged cold cone +300 steps: HIP vs oracle integrals " + " ".join(f"{k}={v:.1e}" for k, v in eo.items()))
+    say("[parity] converged cold cone +300 steps: rel-L2 " + " ".join(f"{k}={v:.2e}" for k, v in errs.items()))
+    assert max(eo.values()) < 1e-3
+    for k, v in errs.items():
+        assert v <= 1e-4, (k, v)
+
+
 def test_deterministic_rerun(cold):
     g1, _ = run_pair(cold, 100)
     g2, _ = run_pair(cold, 100)
