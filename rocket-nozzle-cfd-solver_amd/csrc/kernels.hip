@@ -51,7 +51,13 @@ __global__ void __launch_bounds__(256) k_close(const DevInst* __restrict__ insts
 
 // ------------------------------------------------------------------------------
 // Lagrangian parcels (FS_SPRAY_UPDATE, spray.js:40-121) with the point deposit
-// (VS_SPRAY_DEP, spray.js:131-142) as float atomics into the host cell.
+// (VS_SPRAY_DEP, spray.js:131-142) into the host cell. Parcels crowd into few cells
+// near the injector, where per-parcel float atomics on the same addresses serialise;
+// so each workgroup first sums its parcels' deposits per host cell in an LDS hash table
+// (open addressing, twice as many slots as parcels, so every insert ends) and then adds
+// each distinct cell's sums to the source planes with one float atomic per channel.
+constexpr int DEP_LOG = 9, DEP_SLOTS = 1 << DEP_LOG;
+static_assert(DEP_SLOTS >= 2 * 256, "deposit table: at least two slots per parcel of a workgroup");
 template <typename R>
 __global__ void __launch_bounds__(256) k_spray(const DevInst* __restrict__ insts, int ninst) {
   // instance of this block: the last one whose parcel blocks start at or before it
@@ -67,120 +73,146 @@ __global__ void __launch_bounds__(256) k_spray(const DevInst* __restrict__ insts
   if ((st->flags & 1u) || !c.spray) return;
   const int NP = c.NP;
   const int idx = ((int)blockIdx.x - I.pblk0) * 256 + threadIdx.x;
-  if (idx >= NP) return;
-  const int Nx = I.Nx, Nr = I.Nr;
-  const long long Ns = I.Ns;
-  const R* __restrict__ U1 = static_cast<const R*>(I.U1);
-  float* __restrict__ pc = I.parc;
-  const R dt = R(st->cur[0]);
-  const uint32_t head = st->head, seed = st->seed;
-  const unsigned ui = (unsigned)idx, un = (unsigned)NP;
-  const R ax_ = gld(pc, ui), ay = gld(pc, un + ui), az = gld(pc, 2 * un + ui), aw = gld(pc, 3 * un + ui);
-  const R bx = gld(pc, 4 * un + ui), by = gld(pc, 5 * un + ui), bz = gld(pc, 6 * un + ui), bw = gld(pc, 7 * un + ui);
-  int rel = idx - (int)head;
-  if (rel < 0) rel += NP;
-  R o[8] = {ax_, ay, az, aw, bx, by, bz, bw};
-  R dmo = 0, dpx = 0, dpy = 0;
-  const int perStep = 4;
-  if (rel < perStep) {
-    // new parcel; the liquid left in the replaced one goes to the gas
-    const uint32_t id = (uint32_t)idx ^ (seed * 2654435761u);
-    const int tt = (int)(R(rnd(id, 1u)) * R(255.999f));
-    const float* e1 = I.tab + tt * 4;
-    const float* e2 = I.tab + (256 + tt) * 4;
-    const float* e3 = I.tab + (512 + tt) * 4;
-    const R ang = R(e1[2]) + (R(rnd(id, 2u)) - R(0.5)) * R(2) * R(e2[2]);
-    const R r = fmax(R(e1[1]) + (R(rnd(id, 3u)) - R(0.5)) * R(e2[3]), R(0.05f));
-    const R d = R(e2[1]) * expR((R(rnd(id, 4u)) - R(0.5)) * R(1.4f));
-    o[0] = e1[0]; o[1] = r; o[2] = R(e1[3]) * cosR(ang); o[3] = R(e1[3]) * sinR(ang);
-    o[4] = d; o[5] = R(c.sprayMhat) * dt / R(perStep); o[6] = e2[0]; o[7] = e3[0];
-    if (by > R(0)) { dmo = by; dpx = by * az; dpy = by * aw; }
-  } else if (by > R(0)) {
-    const int gi = min(max((int)floor(ax_), 0), Nx - 1), gj = min(max((int)floor(ay), 0), Nr - 1);
-    const long long kg = (long long)gj * Nx + gi;
-    if (gld(I.code, (unsigned)kg) & C_SOLID) {
-      // born inside solid hardware: walk out into the flow
-      o[0] = ax_ + R(1); o[1] = fmax(ay - R(0.5), R(0.05f)); o[2] = fmax(az, R(0)); o[3] = aw;
-    } else {
-      const float* pr = bz < R(0.5) ? c.fuelVec : c.oxVec;
-      const R pr0 = pr[0], pr1 = pr[1], pr2 = pr[2], pr3 = pr[3];
-      R vx = az, vy = aw;
-      const R m = by, d = bx, dts = dt * R(c.sprayTScale);
-      R dn = d, mn = m;
-      const Prim<R> w = load_prim<R>(U1, Ns, kg);
-      {
-        const R rm = (R(1) - w.Z) * R(c.gas[0]) + w.F * R(c.Rs[0]) + w.O * R(c.Rs[1]) + fmax(w.Z - w.F - w.O, R(0)) * R(c.Rs[2]);
-        const R T = dv(w.p, w.r * rm) * R(c.gas[2]);
-        const R rhoG = w.r * R(1.225f);
-        const R rvx = w.u - vx, rvy = w.v - vy;
-        const R Urel = sq(rvx * rvx + rvy * rvy) * R(287.6f) + R(1e-3f);
-        const R mu = dv(R(1.458e-6f) * T * sq(T), T + R(110.4f));
-        const R Re = dv(rhoG * Urel * d, mu);
-        const R invTau = dv(R(18) * mu * (R(1) + R(0.15f) * powR(fmin(Re, R(1000)), R(0.687f))), pr0 * d * d);
-        const R ex = expR(-dts * invTau);
-        vx = w.u + (vx - w.u) * ex;
-        vy = w.v + (vy - w.v) * ex;
-        const R cp = rm * R(287.05f) * (w.G + R(1));
-        const R Bn = dv(fmax(cp * (T - pr1), R(0)), pr2);
-        const R K = dv(R(8) * dv(mu, R(0.7f)), pr0) * logR(R(1) + Bn) * (R(1) + R(0.27f) * sq(Re));
-        const R d2 = d * d - K * dts;
-        dn = d2 > R(1e-12f) ? sq(d2) : R(0);
-        const R f = dv(dn, d);
-        mn = dn > R(0) ? m * f * f * f : R(0);
-        const R we = dv(rhoG * Urel * Urel * dn, pr3);
-        if (we > R(12) && dn > R(0)) {
-          const R dst = dv(R(12) * pr3, rhoG * Urel * Urel);
-          const R tb = dv(R(3) * dn, Urel) * sq(dv(pr0, rhoG));
-          dn = mixR(dn, fmax(dst, R(2e-6f)), R(1) - expR(-dv(dts, tb)));
+  __shared__ int depKey[DEP_SLOTS];
+  __shared__ float depVal[4][DEP_SLOTS];
+  for (int q = threadIdx.x; q < DEP_SLOTS; q += 256) {
+    depKey[q] = -1;
+    depVal[0][q] = 0.f; depVal[1][q] = 0.f; depVal[2][q] = 0.f; depVal[3][q] = 0.f;
+  }
+  int depH = -1;   // host cell this parcel deposits into (-1: none)
+  float dep[4] = {0.f, 0.f, 0.f, 0.f};
+  if (idx < NP) {
+    const int Nx = I.Nx, Nr = I.Nr;
+    const long long Ns = I.Ns;
+    const R* __restrict__ U1 = static_cast<const R*>(I.U1);
+    float* __restrict__ pc = I.parc;
+    const R dt = R(st->cur[0]);
+    const uint32_t head = st->head, seed = st->seed;
+    const unsigned ui = (unsigned)idx, un = (unsigned)NP;
+    const R ax_ = gld(pc, ui), ay = gld(pc, un + ui), az = gld(pc, 2 * un + ui), aw = gld(pc, 3 * un + ui);
+    const R bx = gld(pc, 4 * un + ui), by = gld(pc, 5 * un + ui), bz = gld(pc, 6 * un + ui), bw = gld(pc, 7 * un + ui);
+    int rel = idx - (int)head;
+    if (rel < 0) rel += NP;
+    R o[8] = {ax_, ay, az, aw, bx, by, bz, bw};
+    R dmo = 0, dpx = 0, dpy = 0;
+    const int perStep = 4;
+    if (rel < perStep) {
+      // new parcel; the liquid left in the replaced one goes to the gas
+      const uint32_t id = (uint32_t)idx ^ (seed * 2654435761u);
+      const int tt = (int)(R(rnd(id, 1u)) * R(255.999f));
+      const float* e1 = I.tab + tt * 4;
+      const float* e2 = I.tab + (256 + tt) * 4;
+      const float* e3 = I.tab + (512 + tt) * 4;
+      const R ang = R(e1[2]) + (R(rnd(id, 2u)) - R(0.5)) * R(2) * R(e2[2]);
+      const R r = fmax(R(e1[1]) + (R(rnd(id, 3u)) - R(0.5)) * R(e2[3]), R(0.05f));
+      const R d = R(e2[1]) * expR((R(rnd(id, 4u)) - R(0.5)) * R(1.4f));
+      o[0] = e1[0]; o[1] = r; o[2] = R(e1[3]) * cosR(ang); o[3] = R(e1[3]) * sinR(ang);
+      o[4] = d; o[5] = R(c.sprayMhat) * dt / R(perStep); o[6] = e2[0]; o[7] = e3[0];
+      if (by > R(0)) { dmo = by; dpx = by * az; dpy = by * aw; }
+    } else if (by > R(0)) {
+      const int gi = min(max((int)floor(ax_), 0), Nx - 1), gj = min(max((int)floor(ay), 0), Nr - 1);
+      const long long kg = (long long)gj * Nx + gi;
+      if (gld(I.code, (unsigned)kg) & C_SOLID) {
+        // born inside solid hardware: walk out into the flow
+        o[0] = ax_ + R(1); o[1] = fmax(ay - R(0.5), R(0.05f)); o[2] = fmax(az, R(0)); o[3] = aw;
+      } else {
+        const float* pr = bz < R(0.5) ? c.fuelVec : c.oxVec;
+        const R pr0 = pr[0], pr1 = pr[1], pr2 = pr[2], pr3 = pr[3];
+        R vx = az, vy = aw;
+        const R m = by, d = bx, dts = dt * R(c.sprayTScale);
+        R dn = d, mn = m;
+        const Prim<R> w = load_prim<R>(U1, Ns, kg);
+        {
+          const R rm = (R(1) - w.Z) * R(c.gas[0]) + w.F * R(c.Rs[0]) + w.O * R(c.Rs[1]) + fmax(w.Z - w.F - w.O, R(0)) * R(c.Rs[2]);
+          const R T = dv(w.p, w.r * rm) * R(c.gas[2]);
+          const R rhoG = w.r * R(1.225f);
+          const R rvx = w.u - vx, rvy = w.v - vy;
+          const R Urel = sq(rvx * rvx + rvy * rvy) * R(287.6f) + R(1e-3f);
+          const R mu = dv(R(1.458e-6f) * T * sq(T), T + R(110.4f));
+          const R Re = dv(rhoG * Urel * d, mu);
+          const R invTau = dv(R(18) * mu * (R(1) + R(0.15f) * powR(fmin(Re, R(1000)), R(0.687f))), pr0 * d * d);
+          const R ex = expR(-dts * invTau);
+          vx = w.u + (vx - w.u) * ex;
+          vy = w.v + (vy - w.v) * ex;
+          const R cp = rm * R(287.05f) * (w.G + R(1));
+          const R Bn = dv(fmax(cp * (T - pr1), R(0)), pr2);
+          const R K = dv(R(8) * dv(mu, R(0.7f)), pr0) * logR(R(1) + Bn) * (R(1) + R(0.27f) * sq(Re));
+          const R d2 = d * d - K * dts;
+          dn = d2 > R(1e-12f) ? sq(d2) : R(0);
+          const R f = dv(dn, d);
+          mn = dn > R(0) ? m * f * f * f : R(0);
+          const R we = dv(rhoG * Urel * Urel * dn, pr3);
+          if (we > R(12) && dn > R(0)) {
+            const R dst = dv(R(12) * pr3, rhoG * Urel * Urel);
+            const R tb = dv(R(3) * dn, Urel) * sq(dv(pr0, rhoG));
+            dn = mixR(dn, fmax(dst, R(2e-6f)), R(1) - expR(-dv(dts, tb)));
+          }
+        }
+        R xi = bw;
+        if (xi > R(0) && ax_ >= xi) {
+          const uint32_t id = (uint32_t)idx ^ (seed * 2654435761u);
+          const R sp = sq(vx * vx + vy * vy), phi = (R(rnd(id, 5u)) - R(0.5)) * R(2) * R(c.fan);
+          vx = sp * cosR(phi);
+          vy = sp * sinR(phi);
+          dn *= R(0.25);
+          xi = R(0);
+        }
+        R x = ax_ + vx * dt, r = ay + vy * dt;
+        if (r < R(0)) { r = -r; vy = -vy; }
+        if (x < R(0)) { x = -x; vx = fabs(vx) * R(0.3f); }
+        auto wallAt = [&](R px, R pr_) {
+          const int qi = (int)floor(px), qj = (int)floor(pr_);
+          if (qi < 0 || qj < 0 || qi >= Nx || qj >= Nr) return false;
+          return (I.code[(long long)qj * Nx + qi] & C_SOLID) != 0;
+        };
+        if (wallAt(x, r)) {
+          const bool hx = wallAt(x, ay), hr = wallAt(ax_, r);
+          if (hx) { vx *= R(-0.3f); x = ax_; }
+          if (hr) { vy *= R(-0.3f); r = ay; }
+          if (!hx && !hr) { x = ax_; r = ay; vx *= R(-0.3f); vy *= R(-0.3f); }
+        }
+        const R dm = m - mn;
+        dmo = dm;
+        dpx = mn * (az - vx) + dm * az;
+        dpy = mn * (aw - vy) + dm * aw;
+        if (x >= R(Nx) || r >= R(Nr) || mn <= R(0)) {
+          o[0] = R(-1); o[1] = R(-1); o[2] = 0; o[3] = 0; o[4] = 0; o[5] = 0; o[6] = 0; o[7] = 0;
+        } else {
+          o[0] = x; o[1] = r; o[2] = vx; o[3] = vy; o[4] = dn; o[5] = mn; o[6] = bz; o[7] = xi;
         }
       }
-      R xi = bw;
-      if (xi > R(0) && ax_ >= xi) {
-        const uint32_t id = (uint32_t)idx ^ (seed * 2654435761u);
-        const R sp = sq(vx * vx + vy * vy), phi = (R(rnd(id, 5u)) - R(0.5)) * R(2) * R(c.fan);
-        vx = sp * cosR(phi);
-        vy = sp * sinR(phi);
-        dn *= R(0.25);
-        xi = R(0);
-      }
-      R x = ax_ + vx * dt, r = ay + vy * dt;
-      if (r < R(0)) { r = -r; vy = -vy; }
-      if (x < R(0)) { x = -x; vx = fabs(vx) * R(0.3f); }
-      auto wallAt = [&](R px, R pr_) {
-        const int qi = (int)floor(px), qj = (int)floor(pr_);
-        if (qi < 0 || qj < 0 || qi >= Nx || qj >= Nr) return false;
-        return (I.code[(long long)qj * Nx + qi] & C_SOLID) != 0;
-      };
-      if (wallAt(x, r)) {
-        const bool hx = wallAt(x, ay), hr = wallAt(ax_, r);
-        if (hx) { vx *= R(-0.3f); x = ax_; }
-        if (hr) { vy *= R(-0.3f); r = ay; }
-        if (!hx && !hr) { x = ax_; r = ay; vx *= R(-0.3f); vy *= R(-0.3f); }
-      }
-      const R dm = m - mn;
-      dmo = dm;
-      dpx = mn * (az - vx) + dm * az;
-      dpy = mn * (aw - vy) + dm * aw;
-      if (x >= R(Nx) || r >= R(Nr) || mn <= R(0)) {
-        o[0] = R(-1); o[1] = R(-1); o[2] = 0; o[3] = 0; o[4] = 0; o[5] = 0; o[6] = 0; o[7] = 0;
-      } else {
-        o[0] = x; o[1] = r; o[2] = vx; o[3] = vy; o[4] = dn; o[5] = mn; o[6] = bz; o[7] = xi;
-      }
+    }
+    for (int q = 0; q < 8; ++q) gst(pc, (unsigned)q * un + ui, (float)o[q]);
+    // deposit into the host of the cell the parcel started in
+    if (!(ax_ < R(0)) && !(dmo == R(0) && dpx == R(0) && dpy == R(0))) {
+      int gi = min(max((int)floor(ax_), 0), Nx - 1), gj = min(max((int)floor(ay), 0), Nr - 1);
+      const uint32_t cg = I.code[(long long)gj * Nx + gi];
+      gi += code_hdx(cg);
+      gj += code_hdy(cg);
+      depH = gj * Nx + gi;
+      dep[0] = (float)(dmo * (R(1) - bz));
+      dep[1] = (float)(dmo * bz);
+      dep[2] = (float)dpx;
+      dep[3] = (float)dpy;
     }
   }
-  for (int q = 0; q < 8; ++q) gst(pc, (unsigned)q * un + ui, (float)o[q]);
-  // deposit into the host of the cell the parcel started in
-  if (!(ax_ < R(0)) && !(dmo == R(0) && dpx == R(0) && dpy == R(0))) {
-    int gi = min(max((int)floor(ax_), 0), Nx - 1), gj = min(max((int)floor(ay), 0), Nr - 1);
-    const uint32_t cg = I.code[(long long)gj * Nx + gi];
-    gi += code_hdx(cg);
-    gj += code_hdy(cg);
-    const long long h = (long long)gj * Nx + gi;
-    float* __restrict__ src = I.src;
-    unsafeAtomicAdd(&src[h], (float)(dmo * (R(1) - bz)));
-    unsafeAtomicAdd(&src[Ns + h], (float)(dmo * bz));
-    unsafeAtomicAdd(&src[2 * Ns + h], (float)dpx);
-    unsafeAtomicAdd(&src[3 * Ns + h], (float)dpy);
+  __syncthreads();
+  if (depH >= 0) {
+    unsigned slot = ((unsigned)depH * 2654435761u) >> (32 - DEP_LOG);
+    for (;;) {
+      const int prev = atomicCAS(&depKey[slot], -1, depH);
+      if (prev == -1 || prev == depH) break;
+      slot = (slot + 1u) & (unsigned)(DEP_SLOTS - 1);
+    }
+    for (int q = 0; q < 4; ++q) atomicAdd(&depVal[q][slot], dep[q]);
+  }
+  __syncthreads();
+  float* __restrict__ src = I.src;
+  const long long Ns = I.Ns;
+  for (int q = threadIdx.x; q < DEP_SLOTS; q += 256) {
+    const int h = depKey[q];
+    if (h < 0) continue;
+    for (int ch = 0; ch < 4; ++ch) unsafeAtomicAdd(&src[ch * Ns + h], depVal[ch][q]);
   }
 }
 
