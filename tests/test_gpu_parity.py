@@ -121,7 +121,7 @@ def test_cold_cone_fp32_vs_fp64(cold):
     assert max(errs.values()) < 1e-3
 
 
-def fp32_noise(setup, steps, label):
+def fp32_noise(setup, steps, label, parcels=None):
     """Relative L2 of the FP32 oracle against the FP64 oracle: the rounding noise any
     FP32 implementation carries at this horizon."""
     o32 = orc.Oracle(setup.Nx, setup.Nr, 32)
@@ -129,6 +129,9 @@ def fp32_noise(setup, steps, label):
     A0, B0 = setup.initial_state()
     for o in (o32, o64):
         o.set_geometry(setup.mask, setup.geoA, setup.geoG, setup.geoM, setup.jWall)
+        if parcels:
+            o.set_parcels(parcels)
+        o.set_injector(setup.injector)
         o.set_state(A0, B0)
         o.step(steps, setup.P)
     a, b = o32.state_f64(), o64.state_f64()
@@ -190,6 +193,49 @@ def test_hot_bell_full_physics():
     bar = max(1e-4, 2 * noise["v"])
     assert errs["v"] <= bar and e64["v"] <= bar, (errs["v"], e64["v"], bar)
     np.testing.assert_allclose(g.readStepInfo(), o.step_info(), rtol=1e-5)
+
+
+@pytest.mark.parametrize("name,parcels", [("hot_bell", None), ("kerolox_injector", 65536)])
+def test_full_size_workload(name, parcels):
+    """The two 768x256 workloads at their full size (the hot bell with full physics, the
+    kerolox injector with 65,536 parcels), 120 steps: FP32 relative L2 of rho, u, p
+    <= 1e-4; v <= 1e-4 or twice the FP32 oracle's own distance from the FP64 oracle (v
+    is near zero over most of the hot-bell domain, where that distance is 2e-4 to 5e-4);
+    the GPU's FP32 v no further from the FP64 oracle than that bar either."""
+    s = SimSetup(getattr(configs, name)())
+    assert (s.Nx, s.Nr) == (768, 256)
+    g, o = run_pair(s, 120, parcels=parcels)
+    noise = fp32_noise(s, 120, f"{name} 768x256", parcels=parcels)
+    G = g.readAll()
+    Ao, Bo = o.state()
+    pg, po = prims(G["A"], G["B"]), prims(Ao, Bo)
+    errs = {k: rel_l2(pg[k], po[k]) for k in pg}
+    say(f"[parity] {name} 768x256: rel-L2 " + " ".join(f"{k}={v:.2e}" for k, v in errs.items()))
+    o64 = orc.Oracle(s.Nx, s.Nr, 64)
+    o64.set_geometry(s.mask, s.geoA, s.geoG, s.geoM, s.jWall)
+    if parcels:
+        o64.set_parcels(parcels)
+    o64.set_injector(s.injector)
+    o64.set_state(*s.initial_state())
+    o64.step(120, s.P)
+    p64 = prims(*o64.state_f64())
+    e64 = {k: rel_l2(pg[k], p64[k]) for k in pg}
+    say(f"[parity] {name} 768x256: GPU FP32 vs oracle FP64 " + " ".join(f"{k}={v:.2e}" for k, v in e64.items()))
+    for k in ("rho", "u", "p"):
+        assert errs[k] <= 1e-4, f"{k} rel-L2 {errs[k]:.3e}"
+    bar = max(1e-4, 2 * noise["v"])
+    assert errs["v"] <= bar and e64["v"] <= bar, (errs["v"], e64["v"], bar)
+    np.testing.assert_allclose(g.readStepInfo(), o.step_info(), rtol=1e-5)
+
+
+@pytest.mark.parametrize("name,parcels,tol", [("hot_bell", None, 1e-9), ("kerolox_injector", 65536, 1e-6)])
+def test_full_size_fp64_logic(name, parcels, tol):
+    """The same workloads in FP64 against the FP64 oracle, 40 steps: the fused kernels'
+    logic at full size (parcels are stored as float32 on the device in both precisions,
+    hence the injector's looser bar)."""
+    s = SimSetup(getattr(configs, name)())
+    g, o = run_pair(s, 40, prec=64, parcels=parcels)
+    compare_state(g, o, tol, f"{name} 768x256 FP64")
 
 
 def test_hot_bell_fp64_logic():
