@@ -38,11 +38,20 @@
 //   source: afterburning    heat = fuel burnt x heat of combustion; mixing limit = shear
 //                           rate; Arrhenius limit linear in p with Ta = 9000 K;
 //                           fuel- / air-limited availability
-//   source: separation      parity unpinned (no closed form for the Zukoski probe)
+//   source: separation      the Zukoski plateau rise 1 + M/2 as the onset, the 1.2x
+//                           ramp, the 4- and 6-cell probe reach, momentum removed at the
+//                           rate rho |u| u; the subsonic band damping
 //   spray update            PCG hash (permutation, uniformity), Stokes drag relaxation,
 //                           d2-law evaporation with the Spalding number, mass balance;
-//                           break-up, impingement fan and wall bounces parity unpinned
-//   rescale, wall row       parity unpinned beyond the GPU comparison
+//                           break-up to the Weber-12 diameter with its time scale
+//                           (d / U) sqrt(rho_l / rho_g); impingement fan (speed kept,
+//                           +-25 degrees, d / 4); axis mirror and 0.3 restitution
+//   rescale                 ambient cells take p fp, T fT at the same gas constant and
+//                           u + du; exhaust cells (Z >= 0.03) unchanged; monotone between;
+//                           mass fractions and G kept
+//   wall row                (u, v, p, sqrt(gamma p / rho)) of the named near-wall cell
+//   derived                 closed forms on constructed states; field 9 against Prandtl's
+//                           mixing length mu_t = rho l^2 |du/dr|; field 13 parity unpinned
 #include <omp.h>
 
 #include <algorithm>

@@ -10,6 +10,14 @@ reference's code:
 - the PCG hash of the spray is a permutation of uint32 with uniform output
 - spray mass balance: gas mass + liquid mass grows only by what is injected
 - quasi-1-D choked mass flow of the cold-gas nozzle against Pc*At/c* (physics.js:86-88)
+- viscous fluxes (axisymmetric Couette), the Werner-Wengle wall law, afterburning rates
+- the separation probe against the Zukoski plateau rise p2/p1 = 1 + M/2, its ramp and
+  reach; the subsonic band damping
+- single droplets: Stokes drag, the d2 law, break-up to the Weber-12 diameter on the
+  (d/U) sqrt(rho_l/rho_g) time scale, the impingement fan, axis and wall bounces
+- the derived display fields on constructed states, the viscosity-ratio field against
+  Prandtl's mixing length
+- the ambient rescale's invariants and the wall-row read-out
 """
 import numpy as np
 import pytest
@@ -522,3 +530,294 @@ def test_derived_fields_closed_forms():
     out = field(0)
     solid = mask.reshape(Nr, Nx) != 0
     assert np.all(out[solid][:, 3] == -1) and np.all(out[~solid][:, 3] >= 0)
+
+
+# ---------------------------------------------------------------- separation probe
+def separation_run(u, p_hi, step_i, bl, chem_p=None):
+    """Uniform flow (rho 1, p 1, velocity u along x) in a closed box with the pressure
+    raised to p_hi from column step_i on; one step with the viscous split step on. The
+    cell (10, 10) carries the boundary-layer band flag when `bl`. Returns its x momentum
+    after the step and the step's dt."""
+    Nx, Nr = 40, 24
+    mask, geoA, geoG, geoM = box_geometry(Nx, Nr)
+    if bl:
+        geoM.reshape(Nr, Nx, 4)[10, 10, 2] += 16
+    i = np.arange(Nx)[None, :] * np.ones((Nr, 1))
+    A, B = state_from_prims(np.ones((Nr, Nx)), np.full((Nr, Nx), u), np.zeros((Nr, Nx)), np.where(i >= step_i, p_hi, 1.0))
+    P = viscous_params()
+    if chem_p is not None:
+        P.chemK[3] = chem_p
+    o = orc.Oracle(Nx, Nr, 64)
+    o.set_geometry(mask, geoA, geoG, geoM)
+    o.set_state(A, B)
+    o.step(1, P)
+    Af, _ = o.state_f64()
+    return Af.reshape(Nr, Nx, 4)[10, 10, 1], float(o.step_info()[0])
+
+
+@pytest.mark.parametrize("p_hi,step_i,frac", [
+    (1.5, 13, "none"),    # below the plateau rise 1 + M/2
+    (2.0, 13, "ramp"),    # inside the smoothstep from 1 + M/2 to 1.2 (1 + M/2)
+    (3.0, 13, "full"),    # above it: the band's momentum is removed at the rate |u|
+    (3.0, 15, "full"),    # only the 6-cell probe reaches the rise
+    (3.0, 17, "none"),    # beyond both probes
+])
+def test_separation_zukoski_plateau(p_hi, step_i, frac):
+    """A supersonic boundary-layer-band cell probes the pressure 4 and 6 cells downstream
+    along its velocity; where it exceeds the turbulent free-interaction plateau rise
+    p2 / p1 = 1 + M / 2 (Zukoski 1967) the cell loses its momentum at the rate rho |u| u,
+    switched on by a smoothstep up to 1.2 times the plateau. The momentum difference of
+    the same step with and without the band flag isolates that term (the flag enters
+    nothing else); the pressure rise starts 3 or more cells downstream, so neither RK
+    stage changes the cell or the probed cells before the split step reads them."""
+    u = 2.0
+    M = u / np.sqrt(1.4)
+    m_bl, dt = separation_run(u, p_hi, step_i, True)
+    m_no, dt2 = separation_run(u, p_hi, step_i, False)
+    assert dt == dt2
+    crit = 1 + 0.5 * M
+    t = np.clip((p_hi - crit) / (0.2 * crit), 0, 1)
+    s = {"none": 0.0, "full": 1.0, "ramp": t * t * (3 - 2 * t)}[frac]
+    if frac == "ramp":
+        assert 0.2 < s < 0.8
+    # dt comes back from the step info as float32 (relative rounding up to 6e-8)
+    assert m_bl - m_no == pytest.approx(-s * dt * u * u, rel=2e-7, abs=1e-15)
+
+
+def test_separation_subsonic_band_damping():
+    """A subsonic band cell moving downstream at near-ambient pressure (below 1.3 times the
+    ambient) is inside a separated region and loses half its momentum rate, rho |u| u / 2,
+    whatever lies downstream; above that pressure it is left alone."""
+    u = 0.5
+    m_bl, dt = separation_run(u, 3.0, 13, True)
+    m_no, _ = separation_run(u, 3.0, 13, False)
+    assert m_bl - m_no == pytest.approx(-0.5 * dt * u * u, rel=2e-7)   # float32 dt
+    m_bl, _ = separation_run(u, 3.0, 13, True, chem_p=0.5)
+    m_no, _ = separation_run(u, 3.0, 13, False, chem_p=0.5)
+    assert m_bl == m_no
+
+
+# ---------------------------------------------------------------- parcels: break-up, fan, bounces
+def parcel_run(parcels, fuel, dx, steps, NP=4096, solid_cols=(0,)):
+    """Parcels (list of (a, b) = ((x, r, u, v), (d, m, type, x_impinge))) in a closed box
+    of still air at 288 K, placed at the top of the ring so that no respawn reaches them.
+    Returns their (PA, PB) after each step, shape (steps, n, 4) each, and the dts."""
+    Nx, Nr = 40, 24
+    mask, geoA, geoG, geoM = box_geometry(Nx, Nr, solid_cols=solid_cols)
+    A, B = state_from_prims(np.ones((Nr, Nx)), np.zeros((Nr, Nx)), np.zeros((Nr, Nx)), np.ones((Nr, Nx)))
+    P = uniform_params()
+    P.inj = 1
+    P.sprayMhat = 0.0
+    P.sprayTScale = dx / 287.6
+    P.fuelVec[:] = fuel
+    P.oxVec[:] = fuel
+    o = orc.Oracle(Nx, Nr, 64)
+    o.set_geometry(mask, geoA, geoG, geoM)
+    o.set_parcels(NP)
+    o.set_injector(np.zeros(256 * 3 * 4, np.float32))
+    o.set_state(A, B)
+    ids = [NP - 1 - k for k in range(len(parcels))]
+    for k, (a, b) in zip(ids, parcels):
+        o.set_parcel(k, a, b)
+    PAs, PBs, dts = [], [], []
+    for _ in range(steps):
+        o.step(1, P)
+        dts.append(float(o.step_info()[0]) * float(np.float32(P.sprayTScale)))
+        PA, PB = o.parcels(NP)
+        PAs.append(PA.reshape(-1, 4)[ids].astype(np.float64))
+        PBs.append(PB.reshape(-1, 4)[ids].astype(np.float64))
+    return np.array(PAs), np.array(PBs), np.array(dts)
+
+
+def weber(d, v, sigma):
+    """Weber number of a drop of diameter d (m) moving at v grid units (x 287.6 m/s, plus the
+    1 mm/s slip floor) through still air of density 1.225 kg/m^3"""
+    U = np.abs(v) * 287.6 + 1e-3
+    return 1.225 * U * U * d / sigma
+
+
+def test_droplet_breakup_to_weber_12():
+    """Break-up above the critical Weber number 12: a drop that sees We < 12 keeps its size;
+    one at We ~ 50 shrinks monotonically towards the diameter at which We = 12 and never
+    below it, keeping its liquid mass (the parcel stands for more, smaller drops). A
+    liquid of 1e10 kg/m^3 makes drag negligible over the run (the slip speed, and with it
+    the stable diameter, stays put); 100 m cells make the steps long enough to watch the
+    relaxation in a few hundred of them; the liquid sits below its boiling point, so
+    nothing evaporates. Each step's Weber number is taken with the slip speed the step
+    started from, the one the break-up rule sees."""
+    sigma = 0.02
+    fuel = (1e10, 1000.0, 1e6, sigma)
+    v0 = 0.1
+    d_small = 0.9 * 12 * sigma / (1.225 * (v0 * 287.6 + 1e-3) ** 2)     # We = 10.8
+    PA, PB, dts = parcel_run([((3.5, 12.5, v0, 0.0), (1e-3, 1e-15, 0.0, 0.0)),
+                              ((3.5, 8.5, v0, 0.0), (d_small, 1e-15, 0.0, 0.0))], fuel, 100.0, 600)
+    d, m, v = PB[:, 0, 0], PB[:, 0, 1], PA[:, 0, 2]
+    v_start = np.concatenate([[np.float32(v0)], v[:-1]])
+    We = weber(d, v_start, sigma)
+    assert weber(1e-3, v0, sigma) > 45
+    assert np.all(np.diff(d) <= 0) and d[-1] < 0.3e-3                   # shrank, monotonically
+    # never below the stable size; once there, break-up stops and the slow drag drift of
+    # the slip speed (1e-4 over the run) may take We just under 12
+    assert np.all(We >= 12 * (1 - 1e-3))
+    assert We[-1] == pytest.approx(12, rel=1e-3)                        # and converged to it
+    np.testing.assert_allclose(m, 1e-15, rtol=1e-6)                     # mass kept
+    np.testing.assert_allclose(PB[:, 1, 0], np.float32(d_small), rtol=1e-7)   # We < 12: untouched
+
+
+def test_droplet_breakup_time_scales_with_density_ratio():
+    """The break-up relaxation time is (d / U) sqrt(rho_l / rho_g) times a constant: four
+    times the liquid density doubles it. Over one step both drops start from the same
+    diameter and slip speed, so -ln(1 - (d0 - d1) / (d0 - d_st)) = dts / t_b must halve."""
+    sigma, d0, v0 = 0.02, 1e-3, 0.1
+    rates = []
+    for rho_l in (1e6, 4e6):
+        PA, PB, dts = parcel_run([((3.5, 12.5, v0, 0.0), (d0, 1e-15, 0.0, 0.0))], (rho_l, 1000.0, 1e6, sigma), 1.0, 1)
+        U = v0 * 287.6 + 1e-3
+        d_st = 12 * sigma / (1.225 * U * U)
+        d1 = PB[0, 0, 0]
+        rates.append(-np.log1p(-(np.float32(d0) - d1) / (np.float32(d0) - d_st)) / dts[0])
+    assert rates[0] / rates[1] == pytest.approx(2.0, rel=2e-4)
+
+
+def test_impingement_fan():
+    """A parcel past its impingement station leaves in a fan: its speed is kept, its
+    direction lies within +-25 degrees of the axis, its diameter drops to a quarter and
+    the station is cleared; 64 parcels cover the fan."""
+    fuel = (1e10, 1000.0, 1e6, 1e3)                      # heavy (no drag), inert, no break-up
+    n, sp0 = 64, 0.3
+    parcels = []
+    for k in range(n):
+        ang = np.deg2rad(-60 + 120 * k / (n - 1))
+        parcels.append(((5.5, 6.5 + 0.1 * (k % 40), sp0 * np.cos(ang), sp0 * np.sin(ang)), (1e-4, 1e-15, 0.0, 5.0)))
+    PA, PB, _ = parcel_run(parcels, fuel, 1.0, 1)
+    v = PA[0, :, 2:4]
+    speed = np.hypot(v[:, 0], v[:, 1])
+    np.testing.assert_allclose(speed, sp0, rtol=1e-6)
+    ang = np.degrees(np.arctan2(v[:, 1], v[:, 0]))
+    assert np.all(np.abs(ang) <= 25 + 1e-6)
+    assert ang.max() - ang.min() > 40
+    np.testing.assert_allclose(PB[0, :, 0], 0.25 * np.float32(1e-4), rtol=1e-6)
+    assert np.all(PB[0, :, 3] == 0)
+
+
+def test_parcel_bounces():
+    """Bounces: at the axis the parcel is mirrored (r -> -r, v_r -> -v_r, speed kept);
+    through the inlet face x = 0 it comes back at 0.3 of its speed; into a solid cell it
+    keeps its position and the velocity component towards the wall reverses at 0.3 (the
+    reference's restitution). The liquid is heavy enough for drag to be negligible."""
+    fuel = (1e10, 1000.0, 1e6, 1e3)
+    dt_probe = parcel_run([((20.5, 12.5, 0.0, 0.0), (1e-4, 1e-15, 0.0, 0.0))], fuel, 1.0, 1, solid_cols=())[2][0]
+    dt = dt_probe / (1.0 / 287.6)                          # the step in grid time units
+    parcels = [((20.5, 0.3 * dt, 0.1, -1.0), (1e-4, 1e-15, 0.0, 0.0)),        # crosses the axis
+               ((0.3 * dt, 10.5, -1.0, 0.05), (1e-4, 1e-15, 0.0, 0.0)),       # crosses x = 0
+               ((20.5, 23.0 - 0.3 * dt, 0.05, 1.0), (1e-4, 1e-15, 0.0, 0.0))]  # runs into the top wall
+    PA, PB, _ = parcel_run(parcels, fuel, 1.0, 1, solid_cols=())
+    (x0, r0, u0, v0), (x1, r1, u1, v1), (x2, r2, u2, v2) = PA[0]
+    assert r0 == pytest.approx(0.7 * dt, rel=1e-5) and v0 == pytest.approx(1.0, rel=1e-6) and u0 == pytest.approx(0.1, rel=1e-6)
+    assert x1 == pytest.approx(0.7 * dt, rel=1e-5) and u1 == pytest.approx(0.3, rel=1e-6) and v1 == pytest.approx(0.05, rel=1e-6)
+    assert r2 == pytest.approx(23.0 - 0.3 * dt, rel=1e-6) and v2 == pytest.approx(-0.3, rel=1e-6)
+    assert u2 == pytest.approx(0.05, rel=1e-6) and x2 == pytest.approx(20.5 + 0.05 * dt, rel=1e-6)
+
+
+# ---------------------------------------------------------------- eddy viscosity, rescale, wall row
+def test_mixing_length_eddy_viscosity_display():
+    """Derived field 9, log10 of the effective over the laminar viscosity, on the shear
+    flow u = a r (v = 0, uniform rho and p), which one step leaves unchanged for the
+    inviscid part: Prandtl's mixing length gives mu_t = rho l^2 |du/dr| (the strain rate
+    of simple shear is a, the divergence 0, so the Ducros sensor is a^2 / (a^2 + 1e-8)),
+    and with no eddy viscosity (l = 0) the ratio is 1. mu_lam is Sutherland's air value
+    scaled by muK, as the Couette test pins it."""
+    from rnozzle.solver import derived_params
+    Nx, Nr, a, muK = 40, 24, 0.02, 400.0
+    mask, geoA, geoG, geoM = box_geometry(Nx, Nr)
+    r = (np.arange(Nr) + 0.5)[:, None] * np.ones((1, Nx))
+    A, B = state_from_prims(np.ones((Nr, Nx)), a * r, np.zeros((Nr, Nx)), np.ones((Nr, Nx)))
+    P = viscous_params(muK=muK)
+    inner = (slice(3, Nr - 4), slice(3, Nx - 4))
+    mu_lam = np.float32(muK) * sutherland_air(288.15)
+    for l in (0.0, 2.0):
+        gm = geoM.reshape(Nr, Nx, 4).copy()
+        gm[..., 3] = l
+        o = orc.Oracle(Nx, Nr, 64)
+        o.set_geometry(mask, geoA, geoG, gm.ravel())
+        o.set_state(A, B)
+        o.step(1, P)
+        v = o.derived(derived_params(9, (0.0, 1.0), P, 3000.0, 288.15, 10.0, 1e-3), 1.0).reshape(Nr, Nx, 4)[..., 0]
+        want = np.log10(1 + l * l * a * (a * a / (a * a + 1e-8)) / mu_lam)
+        np.testing.assert_allclose(v[inner], want, rtol=1e-5, atol=1e-7)
+
+
+def test_ambient_rescale_invariants():
+    """rescaleAmbient(fp, fT, du): pure-ambient cells (exhaust fraction Z = 0) take the new
+    pressure p fp and temperature T fT at the same gas constant (rho -> rho fp / fT) and
+    the flight-speed step u -> u + du; cells at or above the 0.03 threshold keep their
+    state; in between the change is monotone in Z; every cell keeps its mass fractions
+    and G."""
+    Nx, Nr = 40, 24
+    mask, geoA, geoG, geoM = box_geometry(Nx, Nr)
+    i = np.arange(Nx)[None, :] * np.ones((Nr, 1))
+    Z = np.select([i < 10, i < 20, i < 30], [0.0, 0.01, 0.02], 0.05)
+    rho, u, v, p = 1.0 + 0.01 * i, 0.2 + 0.0 * i, 0.05 + 0.0 * i, 0.9 + 0.0 * i
+    A, B = state_from_prims(rho, u, v, p)
+    Bm = B.reshape(Nr, Nx, 4)
+    Bm[..., 0] = Z * rho
+    Bm[..., 1] = 0.5 * Z * rho
+    o = orc.Oracle(Nx, Nr, 64)
+    o.set_geometry(mask, geoA, geoG, geoM)
+    o.set_state(A, Bm.ravel())
+    A0, B0 = o.state_f64()
+    fp, fT, du = 0.7, 1.2, 0.1
+    o.rescale(fp, fT, du, uniform_params())
+    A1, B1 = o.state_f64()
+
+    def prim(Af, Bf):
+        q, s = Af.reshape(Nr, Nx, 4), Bf.reshape(Nr, Nx, 4)
+        r_ = q[..., 0]
+        uu, vv = q[..., 1] / r_, q[..., 2] / r_
+        return r_, uu, vv, (q[..., 3] - 0.5 * r_ * (uu * uu + vv * vv)) / s[..., 2], s[..., 0] / r_, s[..., 1] / r_, s[..., 2]
+
+    r0, u0, v0, p0, Z0, F0, G0 = prim(A0, B0)
+    r1, u1, v1, p1, Z1, F1, G1 = prim(A1, B1)
+    fluid = mask.reshape(Nr, Nx) == 0
+    amb, exh = fluid & (i < 10), fluid & (i >= 30)
+    fp, fT, du = (float(np.float32(x)) for x in (fp, fT, du))     # the factors travel as float32
+    np.testing.assert_allclose(p1[amb] / p0[amb], fp, rtol=1e-12)
+    np.testing.assert_allclose((p1 / r1)[amb] / (p0 / r0)[amb], fT, rtol=1e-12)
+    np.testing.assert_allclose(u1[amb] - u0[amb], du, rtol=1e-12)
+    np.testing.assert_allclose(v1[fluid], v0[fluid], rtol=1e-12)
+    np.testing.assert_allclose(A1.reshape(Nr, Nx, 4)[exh], A0.reshape(Nr, Nx, 4)[exh], rtol=1e-14)
+    np.testing.assert_allclose(Z1[fluid], Z0[fluid], rtol=1e-12, atol=1e-15)
+    np.testing.assert_allclose(F1[fluid], F0[fluid], rtol=1e-12, atol=1e-15)
+    np.testing.assert_allclose(G1[fluid], G0[fluid], rtol=1e-14)
+    ratio = [np.median((p1 / p0)[fluid & (i >= a) & (i < b)]) for a, b in ((0, 10), (10, 20), (20, 30), (30, 40))]
+    assert ratio[0] < ratio[1] < ratio[2] < ratio[3] == pytest.approx(1.0, abs=1e-12)
+
+
+def test_wall_row_readout():
+    """FS_WALLROW: for each column, (u, v, p, c) of the cell in the row jWall[i] names, with
+    c the ideal-gas sound speed sqrt(gamma p / rho), gamma = 1 + 1 / G."""
+    Nx, Nr = 40, 24
+    mask, geoA, geoG, geoM = box_geometry(Nx, Nr)
+    rng = np.random.default_rng(3)
+    rho = 0.5 + rng.random((Nr, Nx))
+    u, v = rng.random((Nr, Nx)) - 0.5, rng.random((Nr, Nx)) - 0.5
+    p = 0.5 + rng.random((Nr, Nx))
+    A, B = state_from_prims(rho, u, v, p)
+    jw = np.zeros((Nx, 4), np.float32)
+    jw[:, 0] = 1 + (np.arange(Nx) * 7) % (Nr - 3)
+    o = orc.Oracle(Nx, Nr, 64)
+    o.set_geometry(mask, geoA, geoG, geoM, jw.ravel())
+    o.set_state(A, B)
+    out = o.wall_row().reshape(Nx, 4).astype(np.float64)
+    Af, Bf = o.state_f64()
+    q = Af.reshape(Nr, Nx, 4)
+    G = Bf.reshape(Nr, Nx, 4)[..., 2]
+    cols = np.arange(1, Nx - 1)                                  # fluid columns of the box
+    j = jw[cols, 0].astype(int)
+    r_ = q[j, cols, 0]
+    uu, vv = q[j, cols, 1] / r_, q[j, cols, 2] / r_
+    pp = (q[j, cols, 3] - 0.5 * r_ * (uu * uu + vv * vv)) / G[j, cols]
+    np.testing.assert_allclose(out[cols, 0], uu, rtol=1e-6, atol=1e-7)
+    np.testing.assert_allclose(out[cols, 1], vv, rtol=1e-6, atol=1e-7)
+    np.testing.assert_allclose(out[cols, 2], pp, rtol=1e-6)
+    np.testing.assert_allclose(out[cols, 3], np.sqrt((1 + 1 / G[j, cols]) * pp / r_), rtol=1e-6)
