@@ -51,7 +51,8 @@
 //                           mass fractions and G kept
 //   wall row                (u, v, p, sqrt(gamma p / rho)) of the named near-wall cell
 //   derived                 closed forms on constructed states; field 9 against Prandtl's
-//                           mixing length mu_t = rho l^2 |du/dr|; field 13 parity unpinned
+//                           mixing length mu_t = rho l^2 |du/dr|; field 13 against the
+//                           Arrhenius burn rate and the fuel the step burns
 #include <omp.h>
 
 #include <algorithm>

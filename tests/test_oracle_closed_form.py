@@ -16,7 +16,7 @@ reference's code:
 - single droplets: Stokes drag, the d2 law, break-up to the Weber-12 diameter on the
   (d/U) sqrt(rho_l/rho_g) time scale, the impingement fan, axis and wall bounces
 - the derived display fields on constructed states, the viscosity-ratio field against
-  Prandtl's mixing length
+  Prandtl's mixing length, the burn-rate field against the burn it shows
 - the ambient rescale's invariants and the wall-row read-out
 """
 import numpy as np
@@ -821,3 +821,42 @@ def test_wall_row_readout():
     np.testing.assert_allclose(out[cols, 1], vv, rtol=1e-6, atol=1e-7)
     np.testing.assert_allclose(out[cols, 2], pp, rtol=1e-6)
     np.testing.assert_allclose(out[cols, 3], np.sqrt((1 + 1 / G[j, cols]) * pp / r_), rtol=1e-6)
+
+
+def test_burn_rate_display_matches_the_burn():
+    """Derived field 13 shows log10(1 + 1e5 x the burning rate per unit mass). On a uniform
+    kinetics-limited gas in injector mode (1440 K, below the 2400 K cap) that rate is what
+    the split step burns, avail k p exp(-9000 / T) with avail = (O + 0.233 (1 - Z)) / s:
+    the display must match both that closed form and the fuel actually consumed in the
+    step, dF / dt (the burn is 1e-6 of the fuel, so reading the display after the step
+    changes nothing measurable)."""
+    from rnozzle.solver import derived_params
+    Nx, Nr = 40, 24
+    mask, geoA, geoG, geoM = box_geometry(Nx, Nr)
+    rho, p, Z, F, O, s, k = 1.0, 5.0, 0.5, 0.4, 0.1, 2.0, 0.01
+    A, B = state_from_prims(np.full((Nr, Nx), rho), np.zeros((Nr, Nx)), np.zeros((Nr, Nx)), np.full((Nr, Nx), p))
+    Bm = B.reshape(Nr, Nx, 4)
+    Bm[..., 0], Bm[..., 1], Bm[..., 3] = Z * rho, F * rho, O * rho
+    P = viscous_params()
+    P.chem = 1
+    P.inj = 1
+    P.chemK[:] = [0.0, s, k, 1.0]
+    P.cap[:] = [2400.0, 2800.0]
+    o = orc.Oracle(Nx, Nr, 64)
+    o.set_geometry(mask, geoA, geoG, geoM)
+    o.set_injector(np.zeros(256 * 3 * 4, np.float32))
+    o.set_state(A, Bm.ravel())
+    o.step(1, P)
+    dt = float(o.step_info()[0])
+    Af, Bf = o.state_f64()
+    inner = (slice(3, Nr - 4), slice(3, Nx - 4))
+    rho1 = Af.reshape(Nr, Nx, 4)[..., 0][inner]
+    F1 = Bf.reshape(Nr, Nx, 4)[..., 1][inner] / rho1
+    burnt_rate = (np.float32(F) - F1) / dt
+    T = p / rho * 288.15
+    avail = (O + 0.233 * (1 - Z)) / s
+    closed = avail * k * p * np.exp(-9000.0 / T)
+    np.testing.assert_allclose(burnt_rate, closed, rtol=2e-5)          # the solver burns at that rate
+    v = o.derived(derived_params(13, (0.0, 1.0), P, 3000.0, 288.15, 10.0, 1e-3), 1.0).reshape(Nr, Nx, 4)[..., 0][inner]
+    np.testing.assert_allclose(v, np.log10(1 + 1e5 * closed), rtol=1e-5)
+    np.testing.assert_allclose(v, np.log10(1 + 1e5 * burnt_rate), rtol=1e-5)
