@@ -102,6 +102,29 @@ def test_mix_matches_alone():
         b.release(jid)
 
 
+def test_batch_with_a_large_instance():
+    """A batch holding an instance above 2048 tiles closes the step with its own kernel
+    after every step, for every resident; each result stays bit-identical to its solve
+    run alone (where the large one also closes every step and the small one folds the
+    step inside its next stage 1: the fold is the same fixed-order sum either way)."""
+    steps = 30
+    setups = [("cold cone 2048", SimSetup(configs.cold_cone(Nx=2048))),
+              ("batch instance 128", SimSetup(configs.batch_instance(np.random.default_rng(3), Nx=128)))]
+    assert (setups[0][1].Nx // 64) * (setups[0][1].Nr // 4) > 2048
+    b = Batch(max_resident=4)
+    ids = [b.submit(s, max_steps=steps, keep_state=True) for _, s in setups]
+    done = run_to_end(b, chunk=10)
+    assert sorted(done) == sorted(ids)
+    for (name, s), jid in zip(setups, ids):
+        assert b.result(jid)["steps"] == steps
+        A, B = b.read_state(jid)
+        _, A1, B1 = alone(s, steps)
+        same = np.array_equal(A, A1) and np.array_equal(B, B1)
+        say(f"[batch] {name} with a 6144-tile neighbour: bit-identical to alone {same}")
+        assert same
+        b.release(jid)
+
+
 def test_graph_and_plain_launches_agree():
     s = SimSetup(configs.hot_bell(Nx=192))
     out = []
