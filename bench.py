@@ -299,8 +299,9 @@ def main():
     if args.calibrate_copy:
         return calibrate_copy(args.calibrate_copy)
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(instances shard across GPUs by design, see the sharding notes)", file=sys.stderr)
+        print(f"bench.py: runs on one GPU; --gpus {args.gpus} is not supported (independent solves "
+              "would shard one queue per GPU; a single grid would split along x with a 2-column halo)",
+              file=sys.stderr)
         sys.exit(2)
 
     if args.workload == "batch":
