@@ -34,6 +34,14 @@ HBM_PEAK_GBS = 8000.0
 STAGE1_BYTES = 92
 STAGE2_BYTES = 128
 STAGE2_SPRAY_EXTRA = 32
+# FP64 runs of the same kernels hold the state in doubles: the 8-plane state streams double
+# (stage 1: 2 x 32 B more, stage 2: 3 x 32 B more), geometry and sources stay float32
+STATE_STREAMS = {1: 2, 2: 3}
+
+
+def stage_bytes(stage, precision):
+    base = STAGE1_BYTES if stage == 1 else STAGE2_BYTES
+    return base + (STATE_STREAMS[stage] * 32 if precision == 64 else 0)
 PARCEL_BYTES = 64 + 32 + 16   # parcel read+write, gas gather, 4-float deposit
 
 # HBM-side bytes per launch of the stage-2 kernel measured with rocprofv3 FETCH_SIZE and
@@ -52,7 +60,7 @@ def cpu_model():
     return platform.processor()
 
 
-def cpu_baseline(setup, budget_s=15.0, budget1_s=5.0):
+def cpu_baseline(setup, budget_s=15.0, budget1_s=5.0, precision=32):
     """The oracle (FP32 CPU restatement, OpenMP) on the same workload for a bounded
     number of steps; cell-updates/s on this host's cores, and on one core beside it."""
     import oracle
@@ -60,7 +68,7 @@ def cpu_baseline(setup, budget_s=15.0, budget1_s=5.0):
     threads = min(threads, 16)
 
     def timed(nthreads, budget, chunk):
-        o = oracle.Oracle(setup.Nx, setup.Nr, 32, threads=nthreads)
+        o = oracle.Oracle(setup.Nx, setup.Nr, precision, threads=nthreads)
         o.set_geometry(setup.mask, setup.geoA, setup.geoG, setup.geoM, setup.jWall)
         o.set_injector(setup.injector)
         A, B = setup.initial_state()
@@ -76,7 +84,7 @@ def cpu_baseline(setup, budget_s=15.0, budget1_s=5.0):
     steps1, dt1 = timed(1, budget1_s, 1)
     cells = setup.Nx * setup.Nr
     return {"value": cells * steps / dt, "unit": "cell-updates/s", "cores": threads, "kind": "port",
-            "sample": f"oracle FP32 on {setup.Nx}x{setup.Nr}, {steps} steps in {dt:.1f} s ({cpu_model()}, "
+            "sample": f"oracle FP{precision} on {setup.Nx}x{setup.Nr}, {steps} steps in {dt:.1f} s ({cpu_model()}, "
                       f"nproc {os.cpu_count()})",
             "one_core": {"value": cells * steps1 / dt1, "sample": f"{steps1} steps in {dt1:.1f} s on 1 thread"}}
 
@@ -278,6 +286,8 @@ def main():
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--workload", default="hot_bell")
+    ap.add_argument("--precision", type=int, default=32, choices=[32, 64],
+                    help="arithmetic of the run (64: the FP64 error-reference run of the same kernels)")
     ap.add_argument("--nx", type=int, default=0, help="override the grid width (other sizes are parity cases)")
     ap.add_argument("--set", action="append", default=[], help="config override key=value (e.g. wallModel=stair)")
     ap.add_argument("--no-cpu", action="store_true")
@@ -327,7 +337,7 @@ def main():
     if over:
         cfg = configs.apply(cfg, over)
     s = SimSetup(cfg)
-    g = GPUSolver()
+    g = GPUSolver(precision=args.precision)
     load_setup(g, s)
     cells = s.Nx * s.Nr
     spray = s.injector is not None
@@ -359,34 +369,38 @@ def main():
     # dominant kernel: the fused stage-2 kernel (stage 1 + split step + spray apply + CFL)
     t2 = ms[2] / ktimed * 1e-3
     t1 = ms[0] / ktimed * 1e-3
-    b2 = cells * (STAGE2_BYTES + (STAGE2_SPRAY_EXTRA if spray else 0))
-    b1 = cells * STAGE1_BYTES
-    tx, ty = tile_shapes(32)[shape]
-    roof = {"bound": "hbm", "kernel": f"k_stage<float,{tx},{ty},2,{'true' if s.P.visc or s.P.chem else 'false'}> "
+    bpc2 = stage_bytes(2, args.precision) + (STAGE2_SPRAY_EXTRA if spray else 0)
+    bpc1 = stage_bytes(1, args.precision)
+    b2, b1 = cells * bpc2, cells * bpc1
+    tx, ty = tile_shapes(args.precision)[shape]
+    rtype = "float" if args.precision == 32 else "double"
+    roof = {"bound": "hbm", "kernel": f"k_stage<{rtype},{tx},{ty},2,{'true' if s.P.visc or s.P.chem else 'false'}> "
                                       "(flux + update + split step + CFL partials, fused)",
             "achieved": b2 / t2 / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": b2 / t2 / 1e9 / HBM_PEAK_GBS,
-            "traffic": MEASURED_TRAFFIC.get((args.workload, f"{s.Nx}x{s.Nr}", shape)) if not over else None,
-            "bytes_per_cell": STAGE2_BYTES + (STAGE2_SPRAY_EXTRA if spray else 0),
+            "traffic": MEASURED_TRAFFIC.get((args.workload, f"{s.Nx}x{s.Nr}", shape))
+            if not over and args.precision == 32 else None,
+            "bytes_per_cell": bpc2,
             "avg_us": t2 * 1e6,
             "stage1": {"achieved": b1 / t1 / 1e9, "frac": b1 / t1 / 1e9 / HBM_PEAK_GBS, "avg_us": t1 * 1e6,
-                       "bytes_per_cell": STAGE1_BYTES}}
+                       "bytes_per_cell": bpc1}}
     out = {
         "metric": "cell-updates/s", "value": value, "unit": "cell-updates/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "fp32", "data": "synthetic (setup from the workload's geometry)",
+        "vs_baseline": None, "dtype": "fp32" if args.precision == 32 else "fp64",
+        "data": "synthetic (setup from the workload's geometry)",
         "config": {"workload": args.workload, "overrides": over, "grid": f"{s.Nx}x{s.Nr}", "cells": cells,
                    "merged_cells": int(s.info.nMerged), "physics": bool(s.P.visc),
                    "afterburning": bool(s.P.chem), "spray": spray, "cfl": round(float(s.P.cfl), 3),
-                   "tile": shape_name(shape),
+                   "tile": shape_name(shape, args.precision),
                    "autotune_us_per_step": None if tune_us is None else
-                   {shape_name(k): round(u, 2) for k, u in enumerate(tune_us)}},
+                   {shape_name(k, args.precision): round(u, 2) for k, u in enumerate(tune_us)}},
         "roofline": roof,
         "kernel_ms": {"steps": ktimed, "stage1": ms[0], "spray": ms[1], "stage2": ms[2], "events_span": ms[3]},
         "sim": {"t_ms": float(si[2]) * s.tScale * 1e3, "dt": float(si[0]), "Pc_bar": integ["Pc"] / 1e5,
                 "mdot_exit": integ["mdot"], "thrust_N": integ["F"], "residual": integ["residual"]},
     }
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(s, args.cpu_seconds)
+        out["cpu_baseline"] = cpu_baseline(s, args.cpu_seconds, precision=args.precision)
     print(json.dumps(out), flush=True)
 
 
