@@ -427,6 +427,13 @@ __global__ void __launch_bounds__(TX* TY, (STAGE == 2 && PHYS) ? 1 : W) k_stage(
         ltv[r] = (idx < S::TW * S::THh && gi >= 0 && gi < Nx && gj >= 0 && gj < Nr) ? gld(I.lt, (unsigned)(gj * Nx + gi)) : 0.f;
       }
     }
+    // this tile's extra faces (faces of its merged cells outside the tile) and the far
+    // cells they reach: workgroup-uniform index loads, issued before the tile's own loads
+    // so that their latency is not added after them
+    const int exBase = I.exStart[lt_];
+    const int nEx = min(I.exStart[lt_ + 1] - exBase, S::MAXE);
+    const int fcBase = I.fcStart[lt_];
+    const int nF = min(I.fcStart[lt_ + 1] - fcBase, S::MAXF);
     // every load of this thread's cells and face areas is issued before any is used
     constexpr int CPT = (S::PW * S::PH + NT - 1) / NT, FPT = (S::NFX + S::NFR + NT - 1) / NT;
     CellRaw<R> raw[CPT];
@@ -470,10 +477,6 @@ __global__ void __launch_bounds__(TX* TY, (STAGE == 2 && PHYS) ? 1 : W) k_stage(
     }
     // extra faces of merged cells outside the tile, and the cells they reach beyond
     // the halo (primitives loaded here in parallel, so no thread waits on them later)
-    const int exBase = I.exStart[lt_];
-    const int nEx = min(I.exStart[lt_ + 1] - exBase, S::MAXE);
-    const int fcBase = I.fcStart[lt_];
-    const int nF = min(I.fcStart[lt_ + 1] - fcBase, S::MAXF);
     for (int e = tid; e < nEx; e += NT) sm.ex[e] = I.exFaces[exBase + e];
     for (int e = tid; e < nF; e += NT) {
       const int2 fcell = I.fcCells[fcBase + e];
