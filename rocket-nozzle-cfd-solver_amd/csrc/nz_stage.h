@@ -352,108 +352,118 @@ __global__ void __launch_bounds__(TX* TY, (STAGE == 2 && PHYS) ? 1 : W) k_stage(
   // (`fused`) or by the close kernel that followed it. Stage 2 takes the time step stage
   // 1 wrote; its first workgroup files the step as completed. Each field is written in a
   // launch in which no workgroup reads it.
-  const bool was = !(st->flags & 1u);
-  bool active = was;
+  bool was = false, active = false, fused = false;
   R cur[4] = {R(0), R(0), R(0), R(0)};
   R dt = R(0), tsel = R(0);
   uint32_t nPart = 0u;
-  bool fused = false;
-  if (STAGE == 1) {
-    nPart = st->nPart;
-    fused = nPart <= FUSED_CLOSE_MAX;
-    tsel = R(st->last[2]);
-    if (was) {
-      const uint32_t f = stop_bits(c, *st, INFINITY);
-      if (f) {
-        active = false;
-        if (lt_ == 0) {
-          // the solve stops here; its last step's residual still decides the stop reason
-          if (fused) fold_groups<NT>(I.part, nPart, tid, sm.foldW, sm.foldR);
-          __syncthreads();
-          if (tid == 0) {
-            if (fused) {
-              double w, r;
-              fold_combine(sm.foldW, sm.foldR, w, r);
-              st->wsnap = (unsigned long long)__double_as_longlong(w);
-              st->resAcc = r;
-              st->resLast = sqrt(r / fmax(st->nGroups, 1.0)) * (double)c.resScale;
+  auto step_state = [&]() {
+    was = !(st->flags & 1u);
+    active = was;
+    if (STAGE == 1) {
+      nPart = st->nPart;
+      fused = nPart <= FUSED_CLOSE_MAX;
+      tsel = R(st->last[2]);
+      if (was) {
+        const uint32_t f = stop_bits(c, *st, INFINITY);
+        if (f) {
+          active = false;
+          if (lt_ == 0) {
+            // the solve stops here; its last step's residual still decides the stop reason
+            if (fused) fold_groups<NT>(I.part, nPart, tid, sm.foldW, sm.foldR);
+            __syncthreads();
+            if (tid == 0) {
+              if (fused) {
+                double w, r;
+                fold_combine(sm.foldW, sm.foldR, w, r);
+                st->wsnap = (unsigned long long)__double_as_longlong(w);
+                st->resAcc = r;
+                st->resLast = sqrt(r / fmax(st->nGroups, 1.0)) * (double)c.resScale;
+              }
+              st->flags |= stop_bits(c, *st, st->resLast);
             }
-            st->flags |= stop_bits(c, *st, st->resLast);
           }
         }
       }
-    }
-  } else {
-    cur[0] = R(st->cur[0]); cur[1] = R(st->cur[1]); cur[2] = R(st->cur[2]); cur[3] = R(st->cur[3]);
-    dt = cur[0];
-    tsel = cur[2];
-    if (active && lt_ == 0 && tid == 0) {
-      st->last[0] = st->cur[0]; st->last[1] = st->cur[1]; st->last[2] = st->cur[2]; st->last[3] = st->cur[3];
-      st->steps += 1;
-      if (c.spray) {
-        st->head = (st->head + 4u) % (uint32_t)c.NP;
-        st->seed = st->seed + 1u;
+    } else {
+      cur[0] = R(st->cur[0]); cur[1] = R(st->cur[1]); cur[2] = R(st->cur[2]); cur[3] = R(st->cur[3]);
+      dt = cur[0];
+      tsel = cur[2];
+      if (active && lt_ == 0 && tid == 0) {
+        st->last[0] = st->cur[0]; st->last[1] = st->cur[1]; st->last[2] = st->cur[2]; st->last[3] = st->cur[3];
+        st->steps += 1;
+        if (c.spray) {
+          st->head = (st->head + 4u) % (uint32_t)c.NP;
+          st->seed = st->seed + 1u;
+        }
+        st->nPart = (uint32_t)(I.tilesX * I.tilesY);
       }
-      st->nPart = (uint32_t)(I.tilesX * I.tilesY);
+    }
+  };
+  // Throughput-tuned shapes (W = 4, many workgroups per CU in turn) read the step state
+  // first; the latency-tuned shapes issue the tile's loads first (below).
+  if constexpr (W >= 4) step_state();
+
+  // ---- 1. primitive tile with ghost halo (FS_PRIM); this tile's extra faces. The tile's
+  // loads depend only on the tile; on the latency-tuned shapes they are issued ahead of
+  // the step state (which decides whether the solve is still running and gives the time
+  // the ghost rules need), so that its round trip overlaps them instead of preceding them.
+  // This thread's own cell: its code word, group geometry and state, for the update
+  // phase; loaded here so that their latency overlaps the tile load and the face loop
+  const int tx = tid % TX, ty = tid / TX;
+  const int ci = i0 + tx, cj = j0 + ty;
+  const bool inDom = ci < Nx && cj < Nr;
+  const long long k = (long long)cj * Nx + ci;
+  uint32_t code = C_IS_SLAVE;
+  float4 gg = make_float4(0.f, 0.f, 0.f, 0.f);
+  R a[4], b[4];
+  if (inDom) {
+    code = gld(I.code, (unsigned)k);
+    gg = gld(I.gg, (unsigned)k);
+    for (int q = 0; q < 4; ++q) { a[q] = gld(Uin + q * Ns, (unsigned)k); b[q] = gld(Uin + (4 + q) * Ns, (unsigned)k); }
+  }
+  // eddy-viscosity length scales of this thread's cells of the viscous ring tile
+  constexpr int LPT = (S::TW * S::THh + NT - 1) / NT;
+  float ltv[LPT];
+  if constexpr (STAGE == 2 && PHYS) {
+#pragma unroll
+    for (int r = 0; r < LPT; ++r) {
+      const int idx = tid + r * NT;
+      const int gi = i0 - 1 + idx % S::TW, gj = j0 - 1 + idx / S::TW;
+      ltv[r] = (idx < S::TW * S::THh && gi >= 0 && gi < Nx && gj >= 0 && gj < Nr) ? gld(I.lt, (unsigned)(gj * Nx + gi)) : 0.f;
     }
   }
+  // this tile's extra faces (faces of its merged cells outside the tile) and the far
+  // cells they reach: workgroup-uniform index loads, issued before the tile's own loads
+  // so that their latency is not added after them
+  const int exBase = I.exStart[lt_];
+  const int nEx = min(I.exStart[lt_ + 1] - exBase, S::MAXE);
+  const int fcBase = I.fcStart[lt_];
+  const int nF = min(I.fcStart[lt_ + 1] - fcBase, S::MAXF);
+  // every load of this thread's cells and face areas is issued before any is used
+  constexpr int CPT = (S::PW * S::PH + NT - 1) / NT, FPT = (S::NFX + S::NFR + NT - 1) / NT;
+  CellRaw<R> raw[CPT];
+  float area[FPT];
+#pragma unroll
+  for (int r = 0; r < CPT; ++r) {
+    const int idx = tid + r * NT;
+    if (idx < S::PW * S::PH) raw[r] = cell_src_load<R>(I, Uin, i0 - 2 + idx % S::PW, j0 - 2 + idx / S::PW);
+  }
+#pragma unroll
+  for (int r = 0; r < FPT; ++r) {
+    const int f = tid + r * NT;
+    area[r] = 0.f;
+    if (f < S::NFX + S::NFR) {
+      const bool dirR = f >= S::NFX;
+      const int g = dirR ? f - S::NFX : f;
+      const int fi = i0 + (dirR ? g % TX : g % (TX + 1)), fj = j0 + (dirR ? g / TX : g / (TX + 1));
+      if (dirR ? (fi < Nx && fj <= Nr) : (fi <= Nx && fj < Nr)) area[r] = face_area(I, dirR, fi, fj);
+    }
+  }
+  if constexpr (W < 4) step_state();
 
   double wloc = 0.0;
   double rloc = 0.0;
   if (active) {
-    // ---- 1. primitive tile with ghost halo (FS_PRIM); this tile's extra faces
-    // this thread's own cell: its code word, group geometry and state, for the update
-    // phase; loaded here so that their latency overlaps the tile load and the face loop
-    const int tx = tid % TX, ty = tid / TX;
-    const int ci = i0 + tx, cj = j0 + ty;
-    const bool inDom = ci < Nx && cj < Nr;
-    const long long k = (long long)cj * Nx + ci;
-    uint32_t code = C_IS_SLAVE;
-    float4 gg = make_float4(0.f, 0.f, 0.f, 0.f);
-    R a[4], b[4];
-    if (inDom) {
-      code = gld(I.code, (unsigned)k);
-      gg = gld(I.gg, (unsigned)k);
-      for (int q = 0; q < 4; ++q) { a[q] = gld(Uin + q * Ns, (unsigned)k); b[q] = gld(Uin + (4 + q) * Ns, (unsigned)k); }
-    }
-    // eddy-viscosity length scales of this thread's cells of the viscous ring tile
-    constexpr int LPT = (S::TW * S::THh + NT - 1) / NT;
-    float ltv[LPT];
-    if constexpr (STAGE == 2 && PHYS) {
-#pragma unroll
-      for (int r = 0; r < LPT; ++r) {
-        const int idx = tid + r * NT;
-        const int gi = i0 - 1 + idx % S::TW, gj = j0 - 1 + idx / S::TW;
-        ltv[r] = (idx < S::TW * S::THh && gi >= 0 && gi < Nx && gj >= 0 && gj < Nr) ? gld(I.lt, (unsigned)(gj * Nx + gi)) : 0.f;
-      }
-    }
-    // this tile's extra faces (faces of its merged cells outside the tile) and the far
-    // cells they reach: workgroup-uniform index loads, issued before the tile's own loads
-    // so that their latency is not added after them
-    const int exBase = I.exStart[lt_];
-    const int nEx = min(I.exStart[lt_ + 1] - exBase, S::MAXE);
-    const int fcBase = I.fcStart[lt_];
-    const int nF = min(I.fcStart[lt_ + 1] - fcBase, S::MAXF);
-    // every load of this thread's cells and face areas is issued before any is used
-    constexpr int CPT = (S::PW * S::PH + NT - 1) / NT, FPT = (S::NFX + S::NFR + NT - 1) / NT;
-    CellRaw<R> raw[CPT];
-    float area[FPT];
-#pragma unroll
-    for (int r = 0; r < CPT; ++r) {
-      const int idx = tid + r * NT;
-      if (idx < S::PW * S::PH) raw[r] = cell_src_load<R>(I, Uin, i0 - 2 + idx % S::PW, j0 - 2 + idx / S::PW);
-    }
-#pragma unroll
-    for (int r = 0; r < FPT; ++r) {
-      const int f = tid + r * NT;
-      area[r] = 0.f;
-      if (f < S::NFX + S::NFR) {
-        const bool dirR = f >= S::NFX;
-        const int g = dirR ? f - S::NFX : f;
-        const int fi = i0 + (dirR ? g % TX : g % (TX + 1)), fj = j0 + (dirR ? g / TX : g / (TX + 1));
-        if (dirR ? (fi < Nx && fj <= Nr) : (fi <= Nx && fj < Nr)) area[r] = face_area(I, dirR, fi, fj);
-      }
-    }
     // the last step's partials are folded while the tile's loads are in flight
     if (STAGE == 1 && fused) fold_groups<NT>(I.part, nPart, tid, sm.foldW, sm.foldR);
 #pragma unroll
