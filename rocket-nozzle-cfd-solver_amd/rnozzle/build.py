@@ -22,6 +22,9 @@ CPP_SOURCES = ["setup.cpp", "errors.cpp"]
 # per-thread arrays stay in private memory rather than being moved into LDS, whose size
 # sets how many stage workgroups share a CU
 DEVICE_FLAGS = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-mllvm", "-disable-promote-alloca-to-lds"]
+# the stage kernels' arguments (instance table pointer, instance count) preloaded into
+# SGPRs at wave launch, so the first table read does not wait on a kernel-argument load
+STAGE_FLAGS = ["-mllvm", "-amdgpu-kernarg-preload-count=2"]
 
 
 def stage_shapes():
@@ -55,7 +58,7 @@ def build_lib(jobs=8):
     # one object per tile shape of the stage kernels
     for name, typ, tx, ty, w in stage_shapes():
         obj = os.path.join(obj_dir, f"stage_{name}.o")
-        cmd = [HIPCC, f"--offload-arch={ARCH}", "-c", *common, *DEVICE_FLAGS, "-munsafe-fp-atomics",
+        cmd = [HIPCC, f"--offload-arch={ARCH}", "-c", *common, *DEVICE_FLAGS, *STAGE_FLAGS, "-munsafe-fp-atomics",
                f"-DNZ_STAGE_NAME={name}", f"-DNZ_STAGE_R={typ}", f"-DNZ_STAGE_TX={tx}", f"-DNZ_STAGE_TY={ty}",
                f"-DNZ_STAGE_W={w}",
                os.path.join(CSRC, "stage_inst.hip"), "-o", obj]
