@@ -5,8 +5,9 @@
 //   k_stage<1>   LDS tile of primitives with a 2-cell ghost halo (FS_PRIM) ->
 //                each x/r face flux computed once (MUSCL + HLLC/HLL, FS_FLUX) ->
 //                merged-group finite-volume update U1 = U0 + dt L(U0) (FS_UPDATE).
-//   k_spray      parcel push, evaporation, break-up; deposits its release with
-//                float atomics into the host cell (FS_SPRAY_UPDATE + FS_SPRAY_DEP).
+//   k_spray      parcel push, evaporation, break-up; deposits its release into the
+//                host cell, summed per cell in LDS, then one float atomic per cell and
+//                channel (FS_SPRAY_UPDATE + FS_SPRAY_DEP).
 //   k_stage<2>   the same flux/update on U1 giving U^{n+1} = (U0 + U1 + dt L(U1))/2,
 //                fused with the viscous / wall / afterburning split step
 //                (FS_VCELL, FS_VFLUX, FS_SOURCE), the spray source apply
