@@ -66,3 +66,40 @@ def test_autotune_leaves_state_and_picks_a_shape():
     b = h.readAll()
     assert np.array_equal(a["A"], b["A"]) and np.array_equal(a["B"], b["B"])
     assert g.stepCount == 40
+
+
+def test_autotune_restores_spray_state():
+    """On a spray solve the autotuner's trial steps move parcels and deposit sources; it
+    restores fields, parcels and step info after each trial, so the solve reads back
+    exactly as before the call."""
+    s = SimSetup(configs.kerolox_injector(Nx=160))
+    g = GPUSolver()
+    g.resize(s.Nx, s.Nr)
+    g.spray.setCount(4096)
+    g.setGeometry({"mask": s.mask, "geoA": s.geoA, "geoG": s.geoG, "geoM": s.geoM, "jWall": s.jWall})
+    g.spray.setInjector({"tab": s.injector})
+    g.setState(*s.initial_state())
+    g.step(20, s.P)
+    before, pa0, si0 = g.readAll(), g.spray.parcels(), g.readStepInfo()
+    g.autotune(s.P, steps=5)
+    after, pa1, si1 = g.readAll(), g.spray.parcels(), g.readStepInfo()
+    assert np.array_equal(before["A"], after["A"]) and np.array_equal(before["B"], after["B"])
+    assert np.array_equal(pa0[0], pa1[0]) and np.array_equal(pa0[1], pa1[1])
+    assert np.array_equal(si0, si1)
+
+
+def test_timed_steps_match_plain_steps():
+    """nz_step_timed (the benchmark's per-kernel timing run) launches the same kernels with
+    events around each: the state it leaves is bit-identical to nz_step's."""
+    import ctypes as C
+    from rnozzle._lib import check
+    s = SimSetup(configs.hot_bell(Nx=192))
+    g1, g2 = GPUSolver(), GPUSolver()
+    for g in (g1, g2):
+        load_setup(g, s)
+    g1.step(30, s.P)
+    ms = (C.c_double * 4)()
+    check(lib().nz_step_timed(g2._h, 30, C.byref(s.P), ms), "nz_step_timed")
+    a, b = g1.readAll(), g2.readAll()
+    assert np.array_equal(a["A"], b["A"]) and np.array_equal(a["B"], b["B"])
+    assert ms[0] > 0 and ms[2] > 0 and ms[3] >= ms[0]
