@@ -470,38 +470,53 @@ def test_converged_integrals(cold):
 
 
 @pytest.mark.timeout(400)
-def test_hot_bell_2ms_integrals():
+@pytest.mark.parametrize("name,t_end_ms,parcels", [("hot_bell", 2.0, None)])
+def test_horizon_integrals(name, t_end_ms, parcels):
     """The headline workload at its own horizon, 2 ms of simulated time (about 8250 steps
-    on 768 x 256): the plume is unsteady by then, so FP32 and FP64 fields have drifted
-    apart (a few percent in relative L2, as the FP32 oracle has from the GPU), but the
-    integrated quantities do not. Exit-plane thrust and exhaust mass flow, throat mass flow
-    and chamber pressure agree within 0.1 %: FP32 against FP64 runs of the same kernels,
-    and the GPU against the oracle over the same step count."""
-    s = SimSetup(configs.hot_bell())
+    on 768 x 256). The plume is unsteady by then, so FP32 and FP64 fields have drifted
+    apart by a few percent in relative L2 (the FP32 oracle sits as far from the GPU), but
+    the integrated quantities have not: exit-plane thrust and exhaust mass flow, throat
+    mass flow and chamber pressure agree within 0.1 %, FP32 against FP64 runs of the same
+    kernels and the GPU against the oracle over the same steps. (The kerolox injector at
+    its 1.5 ms is not a gate here: its float-atomic spray deposit makes every run's
+    trajectory through the unsteady start-up its own, and its integrals then differ run
+    to run by a few 1e-4, too close to the bar for a test that must not flake.)"""
+    s = SimSetup(getattr(configs, name)())
     keys = ("F", "mdot", "mdotThroat", "Pc")
-    runs = {}
+    ORACLE_STEPS = 8250
+    runs, snap = {}, None
     for prec in (32, 64):
         g = GPUSolver(precision=prec)
+        if parcels:
+            g.resize(s.Nx, s.Nr)
+            g.spray.setCount(parcels)
         load_setup(g, s)
         steps = 0
-        while float(g.readStepInfo()[2]) * s.tScale * 1e3 < 2.0:
+        while float(g.readStepInfo()[2]) * s.tScale * 1e3 < t_end_ms:
             g.step(250, s.P)
             steps += 250
+            if prec == 32 and steps == ORACLE_STEPS:
+                snap = g.readAll()
         runs[prec] = (steps, g, g.integrals(s.P))
+        if prec == 32 and snap is None:
+            snap = g.readAll()
     steps, g32, i32 = runs[32]
     _, _, i64 = runs[64]
     e64 = {k: abs(i32[k] / i64[k] - 1) for k in keys}
-    say(f"[parity] hot bell 2 ms ({steps} steps): FP32 vs FP64 integrals " + " ".join(f"{k}={v:.1e}" for k, v in e64.items()))
+    say(f"[parity] {name} {t_end_ms} ms ({steps} steps): FP32 vs FP64 integrals " + " ".join(f"{k}={v:.1e}" for k, v in e64.items()))
     assert max(e64.values()) < 1e-3
     o = orc.Oracle(s.Nx, s.Nr, 32)
     o.set_geometry(s.mask, s.geoA, s.geoG, s.geoM, s.jWall)
+    if parcels:
+        o.set_parcels(parcels)
+    o.set_injector(s.injector)
     o.set_state(*s.initial_state())
-    o.step(steps, s.P)
-    G = g32.readAll()
+    osteps = min(steps, ORACLE_STEPS)
+    o.step(osteps, s.P)
     Ao, Bo = o.state()
-    ig, io = plane_integrals(s, G["A"], G["B"]), plane_integrals(s, Ao, Bo)
+    ig, io = plane_integrals(s, snap["A"], snap["B"]), plane_integrals(s, Ao, Bo)
     eo = {k: abs(ig[k] / io[k] - 1) for k in keys}
-    say(f"[parity] hot bell 2 ms: GPU vs oracle integrals " + " ".join(f"{k}={v:.1e}" for k, v in eo.items()))
+    say(f"[parity] {name} after {osteps} steps: GPU vs oracle integrals " + " ".join(f"{k}={v:.1e}" for k, v in eo.items()))
     assert max(eo.values()) < 1e-3
 
 
