@@ -64,8 +64,9 @@ def cpu_baseline(setup, budget_s=15.0, budget1_s=5.0, precision=32):
     """The oracle (FP32 CPU restatement, OpenMP) on the same workload for a bounded
     number of steps; cell-updates/s on this host's cores, and on one core beside it."""
     import oracle
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count() or 1
-    threads = min(threads, 16)
+    # the host's CPU share: OMP_NUM_THREADS where the environment sets it, else every
+    # core this process may run on (nproc)
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0)) or 1
 
     def timed(nthreads, budget, chunk):
         o = oracle.Oracle(setup.Nx, setup.Nr, precision, threads=nthreads)

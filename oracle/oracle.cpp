@@ -26,9 +26,19 @@
 // Templated on the arithmetic type: float reproduces the reference's FP32
 // arithmetic, double is the FP64 error reference. OpenMP over rows.
 //
-// How each pass is pinned (tests/test_oracle_closed_form.py, tests/test_setup_golden.py):
+// How each pass is pinned (tests/test_oracle_closed_form.py, tests/test_oracle_pins.py,
+// tests/test_setup_golden.py):
 //   prim / flux / update    exact Riemann solutions (Sod, Toro test 2), closed-box
 //                           conservation, free-stream preservation on cut cells
+//   face flux               Toro's HLLC (second form) and the Einfeldt HLLE flux on random
+//                           state pairs at blend weight 0 and 1; the weight on constructed
+//                           tangential pressure jumps (ramp, both ends, wall neighbour);
+//                           an isolated contact kept exactly
+//   wall pressure           the exact reflected-shock / double-rarefaction pressure of the
+//                           symmetric Riemann problem, and the cavitation floor
+//   ghost cells             inlet from the isentropic stagnation relations (ramp, absorbing
+//                           speed); far field and outflow, every branch, through the
+//                           boundary face's flux; a supersonic stream kept uniform
 //   CFL / step info         brute force over the definition
 //   inlet + whole step      choked mass flow against Pc At / c*
 //   vcell / vflux           axisymmetric Couette: momentum mu a / r and energy 2 mu a^2
@@ -41,6 +51,11 @@
 //   source: separation      the Zukoski plateau rise 1 + M/2 as the onset, the 1.2x
 //                           ramp, the 4- and 6-cell probe reach, momentum removed at the
 //                           rate rho |u| u; the subsonic band damping
+//   spray deposit / apply   a drop in a merged slave cell: mass, both momenta, energy
+//                           (latent and drag work), G mixing and fractions into its group;
+//                           gas + liquid mass and momentum conserved
+//   spray respawn           every draw against a literal table of the published pcg_hash;
+//                           element frequencies against the slot counts; jitter bounds
 //   spray update            PCG hash (permutation, uniformity), Stokes drag relaxation,
 //                           d2-law evaporation with the Spalding number, mass balance;
 //                           break-up to the Weber-12 diameter with its time scale
@@ -1106,6 +1121,37 @@ void set_state(Oracle<T>& o, const float* A, const float* B) {
 template <class T>
 void set_inj(Oracle<T>& o, const float* tab) { copy_in(o.tab, tab, 256 * 3 * 4); }
 
+// The first stage of the next step (CFL, FS_PRIM, FS_FLUX_X/R, FS_UPDATE stage 1) from the
+// current state, leaving the oracle as it was: the stage-1 state, the face-flux textures
+// (per face F = (mass, x mom, r mom, energy), FY = (rho Z u, rho F u, uf Gf, uf),
+// FO = (rho O u, 0, 0, 0), unweighted, as solver.js:246-255 writes them) and the step info.
+template <class T>
+void stage1_probe(Oracle<T>& o, const nz_params& P, double* A1, double* B1, double* Fx, double* Fr, double* info) {
+  T saved[2][4];
+  std::memcpy(saved, o.step_, sizeof(saved));
+  const int savedIdx = o.stepIdx;
+  const int c0 = o.cur;
+  std::vector<T> tA(o.SA[c0].size()), tB(o.SB[c0].size());
+  o.cfl(o.SA[c0], o.SB[c0], P);
+  for (int k = 0; k < 4; ++k) info[k] = (double)o.step_[o.stepIdx][k];
+  o.prim_pass(o.SA[c0], o.SB[c0], 1, P);
+  o.flux_pass(P);
+  o.update_pass(o.SA[c0], o.SB[c0], o.SA[c0], o.SB[c0], 1, tA, tB, P);
+  for (size_t k = 0; k < tA.size(); ++k) { A1[k] = (double)tA[k]; B1[k] = (double)tB[k]; }
+  const size_t NL = (size_t)(o.Nx + 1) * (o.Nr + 1);
+  for (size_t f = 0; f < NL; ++f)
+    for (int q = 0; q < 4; ++q) {
+      Fx[f * 12 + q] = (double)o.FxA[f * 4 + q];
+      Fx[f * 12 + 4 + q] = (double)o.FxB[f * 4 + q];
+      Fx[f * 12 + 8 + q] = (double)o.FxC[f * 4 + q];
+      Fr[f * 12 + q] = (double)o.FrA[f * 4 + q];
+      Fr[f * 12 + 4 + q] = (double)o.FrB[f * 4 + q];
+      Fr[f * 12 + 8 + q] = (double)o.FrC[f * 4 + q];
+    }
+  std::memcpy(o.step_, saved, sizeof(saved));
+  o.stepIdx = savedIdx;
+}
+
 }  // namespace
 
 #define DISPATCH(h, CALL) ((h)->prec == 64 ? ((h)->d.CALL) : ((h)->f.CALL))
@@ -1190,6 +1236,13 @@ void oracle_read_parcels(void* p, float* PA, float* PB) {
   if (h->prec == 64) { copy_out(PA, h->d.PA[h->d.spCur], h->d.PA[0].size()); copy_out(PB, h->d.PB[h->d.spCur], h->d.PB[0].size()); }
   else { copy_out(PA, h->f.PA[h->f.spCur], h->f.PA[0].size()); copy_out(PB, h->f.PB[h->f.spCur], h->f.PB[0].size()); }
 }
+void oracle_read_parcels_f64(void* p, double* PA, double* PB) {
+  Handle* h = static_cast<Handle*>(p);
+  auto get = [&](auto& o) {
+    for (size_t k = 0; k < o.PA[0].size(); ++k) { PA[k] = (double)o.PA[o.spCur][k]; PB[k] = (double)o.PB[o.spCur][k]; }
+  };
+  if (h->prec == 64) get(h->d); else get(h->f);
+}
 // FS_DERIVED display fields of the current state (velScale: U_REF)
 void oracle_derived(void* p, const nz_derived_params* d, double velScale, float* out) {
   Handle* h = static_cast<Handle*>(p);
@@ -1209,4 +1262,12 @@ void oracle_set_parcel(void* p, int k, const float* a, const float* b) {
 }
 // pieces of one pass, for closed-form tests of the restatement itself
 int oracle_pcg(uint32_t v) { return (int)Oracle<float>::pcg(v); }
+float oracle_rnd(uint32_t a, uint32_t n) { return Oracle<float>::rnd(a, n); }
+// stage 1 of the next step without committing it (A1, B1: Nx*Nr*4; Fx, Fr: (Nx+1)*(Nr+1)*12;
+// info: the step info of that step)
+void oracle_stage1_probe(void* p, const nz_params* P, double* A1, double* B1, double* Fx, double* Fr, double* info) {
+  Handle* h = static_cast<Handle*>(p);
+  if (h->prec == 64) stage1_probe(h->d, *P, A1, B1, Fx, Fr, info);
+  else stage1_probe(h->f, *P, A1, B1, Fx, Fr, info);
+}
 }

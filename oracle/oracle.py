@@ -49,9 +49,12 @@ def lib():
             "oracle_read_t2": (None, [vp, fp]),
             "oracle_wall_row": (None, [vp, fp]),
             "oracle_read_parcels": (None, [vp, fp, fp]),
+            "oracle_read_parcels_f64": (None, [vp, dp, dp]),
             "oracle_set_parcel": (None, [vp, C.c_int, fp, fp]),
             "oracle_derived": (None, [vp, C.c_void_p, C.c_double, fp]),
             "oracle_pcg": (C.c_int, [C.c_uint32]),
+            "oracle_rnd": (C.c_float, [C.c_uint32, C.c_uint32]),
+            "oracle_stage1_probe": (None, [vp, C.c_void_p, dp, dp, dp, dp, dp]),
         }
         for n, (r, a) in sig.items():
             f = getattr(L, n)
@@ -141,10 +144,31 @@ class Oracle:
         lib().oracle_derived(self._h, C.addressof(d), vel_scale, _f(out))
         return out
 
+    def stage1_probe(self, P):
+        """Stage 1 of the next step without committing it: (A1, B1) as (Nr, Nx, 4), the x-
+        and r-face flux textures as (Nr + 1, Nx + 1, 12) = (F, FY, FO), and the step info."""
+        Nx, Nr = self.Nx, self.Nr
+        A1 = np.zeros(Nx * Nr * 4)
+        B1 = np.zeros_like(A1)
+        Fx = np.zeros((Nx + 1) * (Nr + 1) * 12)
+        Fr = np.zeros_like(Fx)
+        info = np.zeros(4)
+        dp = C.POINTER(C.c_double)
+        lib().oracle_stage1_probe(self._h, C.addressof(P), *(a.ctypes.data_as(dp) for a in (A1, B1, Fx, Fr, info)))
+        return (A1.reshape(Nr, Nx, 4), B1.reshape(Nr, Nx, 4), Fx.reshape(Nr + 1, Nx + 1, 12),
+                Fr.reshape(Nr + 1, Nx + 1, 12), info)
+
     def parcels(self, n):
         PA = np.zeros(n * 4, np.float32)
         PB = np.zeros(n * 4, np.float32)
         lib().oracle_read_parcels(self._h, _f(PA), _f(PB))
+        return PA, PB
+
+    def parcels_f64(self, n):
+        PA = np.zeros(n * 4)
+        PB = np.zeros(n * 4)
+        dp = C.POINTER(C.c_double)
+        lib().oracle_read_parcels_f64(self._h, PA.ctypes.data_as(dp), PB.ctypes.data_as(dp))
         return PA, PB
 
     def __del__(self):

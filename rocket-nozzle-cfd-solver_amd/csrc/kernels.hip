@@ -37,7 +37,10 @@ __global__ void __launch_bounds__(256) k_close(const DevInst* __restrict__ insts
   if (n >= ninst) return;
   const DevInst& J = insts[n];
   InstState* s = J.st;
-  if (s->flags & 1u) return;
+  if (s->flags & 1u) {
+    if (threadIdx.x == 0) s->flagsSeen = s->flags;
+    return;
+  }
   fold_groups<256>(J.part, s->nPart, threadIdx.x, W, R);
   __syncthreads();
   double w, r;
@@ -47,6 +50,7 @@ __global__ void __launch_bounds__(256) k_close(const DevInst* __restrict__ insts
     s->resAcc = r;
     s->resLast = sqrt(r / fmax(s->nGroups, 1.0)) * (double)J.c.resScale;
     s->flags |= stop_bits(J.c, *s, s->resLast);
+    s->flagsSeen = s->flags;
   }
 }
 

@@ -75,6 +75,9 @@ struct InstState {
   double nGroups;  // number of fluid groups (residual normalisation)
   uint32_t flags;  // bit 0: instance inactive (skipped); stopped on bit 1: the residual, 2: time, 3: step count
   uint32_t nPart;  // tile partials (wave speed, residual) the last stage-2 launch wrote (or the CFL set-up)
+  uint32_t flagsSeen;  // `flags` as the last stage 2 (or close kernel) read them: stage 1, whose first
+                       // workgroup writes `flags`, reads this copy instead
+  uint32_t pad_;
 };
 
 // The next step's stage 1 folds the tile partials of the last stage 2 into the time step

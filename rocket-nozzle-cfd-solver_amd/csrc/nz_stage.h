@@ -351,13 +351,15 @@ __global__ void __launch_bounds__(TX* TY, (STAGE == 2 && PHYS) ? 1 : W) k_stage(
   // of the partials the last stage 2 wrote, folded here by wave 0 of every workgroup
   // (`fused`) or by the close kernel that followed it. Stage 2 takes the time step stage
   // 1 wrote; its first workgroup files the step as completed. Each field is written in a
-  // launch in which no workgroup reads it.
+  // launch in which no workgroup reads it: stage 1's first workgroup sets the stop bits in
+  // `flags`, which stage 2 reads, while stage 1 reads `flagsSeen`, the copy that stage 2's
+  // first workgroup (or the close kernel) writes.
   bool was = false, active = false, fused = false;
   R cur[4] = {R(0), R(0), R(0), R(0)};
   R dt = R(0), tsel = R(0);
   uint32_t nPart = 0u;
   auto step_state = [&]() {
-    was = !(st->flags & 1u);
+    was = !((STAGE == 1 ? st->flagsSeen : st->flags) & 1u);
     active = was;
     if (STAGE == 1) {
       nPart = st->nPart;
@@ -388,6 +390,7 @@ __global__ void __launch_bounds__(TX* TY, (STAGE == 2 && PHYS) ? 1 : W) k_stage(
       cur[0] = R(st->cur[0]); cur[1] = R(st->cur[1]); cur[2] = R(st->cur[2]); cur[3] = R(st->cur[3]);
       dt = cur[0];
       tsel = cur[2];
+      if (lt_ == 0 && tid == 0) st->flagsSeen = st->flags;
       if (active && lt_ == 0 && tid == 0) {
         st->last[0] = st->cur[0]; st->last[1] = st->cur[1]; st->last[2] = st->cur[2]; st->last[3] = st->cur[3];
         st->steps += 1;

@@ -23,7 +23,7 @@ def lib():
     """The loaded library with every header signature bound (raises if absent)."""
     global _lib
     if _lib is None:
-        path = os.environ.get("RNOZZLE_LIB", LIB_PATH)
+        path = LIB_PATH
         if not os.path.exists(path):
             raise NozzleError(f"native library not built: {path} (run __graft_entry__.build())")
         L = C.CDLL(path)

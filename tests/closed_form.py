@@ -102,6 +102,33 @@ def box_geometry(Nx, Nr, solid_cols=(0,), solid_right=True, solid_top=True):
         solid[:, Nx - 1] = True
     if solid_top:
         solid[Nr - 1, :] = True
+    return cells_geometry(solid)
+
+
+def merge_cells(geoG, geoM, Nx, Nr, slave, host):
+    """Merge full cell `slave` = (i, j) into its 4-neighbour `host` (cutcell.js:121-142,
+    164-184 encoding): both members carry the group sums (V, Am, Wx, Wr) in geoG, the slave
+    its host offset in geoM.xy, the host the slave's direction bit in geoM.z (1: west of the
+    host, 2: east, 4: south, 8: north)."""
+    G = geoG.reshape(Nr, Nx, 4)
+    M = geoM.reshape(Nr, Nx, 4)
+    (si, sj), (hi, hj) = slave, host
+    assert abs(si - hi) + abs(sj - hj) == 1
+    group = G[sj, si].astype(np.float64) + G[hj, hi].astype(np.float64)
+    G[sj, si] = group
+    G[hj, hi] = group
+    M[sj, si, 0] = hi - si
+    M[sj, si, 1] = hj - sj
+    bit = 1 if si < hi else 2 if si > hi else 4 if sj < hj else 8
+    M[hj, hi, 2] = float(int(M[hj, hi, 2]) | bit)
+
+
+def cells_geometry(solid):
+    """Geometry of full square cells (no cut cells) with the given solid mask (Nr x Nx):
+    volumes j + 1/2, r-weighted face areas, wall vectors by closure, in the reference's
+    layout (cutcell.js:144-184). Returns mask, geoA, geoG, geoM (flat float32 arrays)."""
+    solid = np.asarray(solid, bool)
+    Nr, Nx = solid.shape
     j = np.arange(Nr)[:, None] * np.ones((1, Nx))
     V = np.where(solid, 0.0, j + 0.5)
     Am = np.where(solid, 0.0, 1.0)
