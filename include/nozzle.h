@@ -193,17 +193,32 @@ int nz_read_derived(nz_solver* s, const nz_derived_params* d, float* out);
 /* Integrated quantities, summed on the device in FP64.
  * exitPerformance (simsetup.js:314-337), chamberPerformance (239-250),
  * cellPrimSI at the chamber probe (338-348, app.js:537). */
-typedef struct nz_probes { int32_t iExit, jExit, iThroat, iChamber; double dx; } nz_probes;
+typedef struct nz_probes {
+  int32_t iExit, jExit, iThroat, iChamber;  /* setup.iExit, jExit, iThroat, iChamber */
+  double dx;                                /* cell size, m */
+  /* exact != 0: the parameter-block values below, at the double precision the reference's
+   * P object holds them (simsetup.js:257-311); exact == 0: the float uniforms of the
+   * nz_params passed to nz_integrals_read, and a chamberPerformance target of 1 */
+  int32_t exact;
+  double pa;                                /* P.amb[3], normalised ambient pressure */
+  double Rs[3], RaN;                        /* P.Rs, P.RaN (chamber-probe gas constant) */
+  double sprayMdot;                         /* P.spray.mdot, kg/s (0 without spray: target 1) */
+} nz_probes;
 typedef struct nz_integrals {
   double F;            /* exit-plane thrust, N */
   double mdot;         /* exit-plane exhaust mass flow, kg/s (simsetup.js:324) */
   double Me, pe, Isp;  /* exit Mach, mean exit pressure (Pa), Isp (s) */
-  double mdotThroat;   /* throat-plane total mass flow, kg/s */
+  double mdotThroat;   /* throat-plane total mass flow, kg/s (chamberPerformance.mdot) */
   double Pc, Tc;       /* chamber probe pressure (Pa), temperature (K) */
   double mass, energy; /* sum over fluid groups of rho*V, E*V (normalised) */
   double residual;     /* L2 of d(rho)/dt over fluid groups, last step, per unit grid time */
+  double unburntOx;    /* chamberPerformance.unburntOx: throat-plane rho*O flow / target */
+  double fuelVapour;   /* chamberPerformance.fuelVapour: throat-plane rho*F flow / target */
 } nz_integrals;
 int nz_set_probes(nz_solver* s, const nz_probes* pr);
+/* exitPerformance + chamberPerformance + cellPrimSI(iChamber, 0) of the current state
+ * (simsetup.js:314-337, 239-250, 338-348; app.js:534-539, 597), summed on the device in
+ * FP64 in a fixed order (the same bits on every call) */
 int nz_integrals_read(nz_solver* s, const nz_params* P, nz_integrals* out);
 
 /* ------------------------------------------------------------------------ */
@@ -213,8 +228,8 @@ int nz_integrals_read(nz_solver* s, const nz_params* P, nz_integrals* out);
 /* request loop). Every resident solve of any grid size advances in the     */
 /* same launches (one per kernel per step over all residents' tiles), from  */
 /* the same kernels as nz_step, so a solve's result does not depend on what */
-/* else is resident (bit-identical without spray; the spray's float-atomic  */
-/* deposit order is not deterministic in either path). Jobs are admitted    */
+/* else is resident (bit-identical, the spray included: its deposit sums   */
+/* fixed-point integers, whose order leaves no trace). Jobs are admitted    */
 /* first-fit in submission order while the device-memory budget and the     */
 /* resident limit allow; a solve retires when it meets its stop test, its   */
 /* memory is freed and queued jobs take its place at the next run call.     */
@@ -315,6 +330,7 @@ typedef struct nz_config {
   double convAngle;           /* converging half-angle, deg (reference: 35, physics.js:129) */
   double ambP, ambT;          /* > 0: ambient pressure (Pa) / temperature (K) instead of ISA(alt) */
   int32_t turbulence;         /* 0: eddy-viscosity length scale l = 0 everywhere */
+  int32_t deviceGeometry;     /* 1: the cut-cell geometry is built on the GPU (nz_cutcell_build) */
 } nz_config;
 
 typedef struct nz_setup nz_setup;
@@ -351,6 +367,30 @@ int nz_setup_get_geometry(const nz_setup* s, uint8_t* mask, float* geoA, float* 
 int nz_setup_initial_state(const nz_setup* s, float* A, float* B);
 /* setup.injector.tab (simsetup.js:181-237); returns -1 if not injector mode */
 int nz_setup_get_injector(const nz_setup* s, float* tab);
+/* probes and double-precision parameter values of the setup for nz_set_probes (exact = 1) */
+int nz_setup_get_probes(const nz_setup* s, nz_probes* out);
+
+/* Cut-cell geometry on the GPU: buildCutGeometry (cutcell.js:14-186) as kernels, bit for
+ * bit the reference's (and the host restatement's) output. The contour enters as the
+ * reference's fine radius table plus its polyline (for the look-up past the table's end). */
+typedef struct nz_cutcell_in {
+  int32_t Nx, Nr;
+  double dx, tw, xBase, bodyR;      /* cell size, wall thickness, base x, body radius (m) */
+  int32_t base, stair;              /* base body present; staircase walls (no cut cells) */
+  const float* Rtab;                /* Nx*8+1: inner-wall radius (cells) every 1/8 cell, -1e9 past
+                                       the exit (cutcell.js:19-27) */
+  int32_t npts;
+  const double *xs, *rs;            /* contour polyline (m), radiusAt (physics.js:190-197) */
+  double Rc, Re, xe, cavD, cavW;    /* chamber / exit radius, exit x, cavity pocket (m) */
+  int32_t nsolids;
+  const double* solids;             /* nsolids x (x0, x1, r0, r1), m (chamber hardware) */
+} nz_cutcell_in;
+/* the setup's cut-cell input; its pointers stay valid while the setup lives */
+int nz_setup_get_cutcell_input(const nz_setup* s, nz_cutcell_in* out);
+/* host arrays in and out, as nz_setup_get_geometry fills them; counts: NULL or (nCut, nMerged).
+ * Allocates its device buffers for the call and frees them. */
+int nz_cutcell_build(const nz_cutcell_in* in, uint8_t* mask, float* geoA, float* geoG, float* geoM, float* kappa,
+                     float* Ax, float* Ar, int32_t* counts);
 /* HUD read-outs on the device's near-wall row (nz_draw_wall_row + NZ_FIELD_WALLROW:
  * Nx x (u, v, p, c), normalised). wallSeparation (simsetup.js:352-373): returns 1
  * and fills *out when the bell has separated, 0 while attached. */

@@ -47,7 +47,7 @@ size_t inst_bytes(int Nx, int Nr, int prec, int NP) {
   return Ns * 8 * R * 2                                   // U0, U1
          + (size_t)Nr * (Nx + 1) * 4 + (size_t)(Nr + 1) * Nx * 4  // ax, ar
          + N * (16 + 4 + 4)                               // gg, code, lt
-         + Ns * 16                                        // spray sources
+         + Ns * 32                                        // spray sources (4 x int64)
          + (size_t)NP * 32                                // parcels
          + tiles * (16 + 8) + (size_t)Nx * 32 + 4096;     // partials, starts, wall row, small
 }
@@ -75,7 +75,7 @@ int inst_alloc(Inst& I, int Nx, int Nr, int prec, InstState* st, int shape) {
   HIPCHK(hipMalloc(&I.code, (size_t)N * 4), "hipMalloc code");
   HIPCHK(hipMalloc(&I.lt, (size_t)N * 4), "hipMalloc lt");
   HIPCHK(hipMalloc(&I.jw, (size_t)Nx * 16), "hipMalloc jWall");
-  HIPCHK(hipMalloc(&I.src, (size_t)I.Ns * 16), "hipMalloc src");
+  HIPCHK(hipMalloc(&I.src, (size_t)I.Ns * 32), "hipMalloc src");
   HIPCHK(hipMalloc(&I.tab, 256 * 3 * 16), "hipMalloc tab");
   I.partCap = part_capacity(Nx, Nr, prec);
   HIPCHK(hipMalloc(&I.part, I.partCap * sizeof(double2)), "hipMalloc partials");
@@ -86,7 +86,7 @@ int inst_alloc(Inst& I, int Nx, int Nr, int prec, InstState* st, int shape) {
   HIPCHK(hipMalloc(&I.fcCells, sizeof(int2)), "hipMalloc far cells");
   HIPCHK(hipMalloc(&I.exFlux, 16 * 8), "hipMalloc extra fluxes");
   HIPCHK(hipMalloc(&I.wallrow, (size_t)Nx * 16), "hipMalloc wall row");
-  HIPCHK(hipMalloc(&I.dint, 16 * 8), "hipMalloc integrals");
+  HIPCHK(hipMalloc(&I.dint, (16 + 2 * (size_t)nzk::INTEG_BLOCKS_MAX) * 8), "hipMalloc integrals");
   if (st) {
     I.st = st;
     I.ownSt = false;
@@ -101,7 +101,7 @@ int inst_alloc(Inst& I, int Nx, int Nr, int prec, InstState* st, int shape) {
   HIPCHK(hipMemsetD32((hipDeviceptr_t)I.code, (int)kZeroOffset, (size_t)N), "memset code");
   HIPCHK(hipMemset(I.lt, 0, (size_t)N * 4), "memset");
   HIPCHK(hipMemset(I.jw, 0, (size_t)Nx * 16), "memset");
-  HIPCHK(hipMemset(I.src, 0, (size_t)I.Ns * 16), "memset");
+  HIPCHK(hipMemset(I.src, 0, (size_t)I.Ns * 32), "memset");
   HIPCHK(hipMemset(I.tab, 0, 256 * 3 * 16), "memset");
   HIPCHK(hipMemset(I.exStart, 0, (tiles + 1) * sizeof(int)), "memset");
   HIPCHK(hipMemset(I.fcStart, 0, (tiles + 1) * sizeof(int)), "memset");
@@ -353,6 +353,8 @@ int inst_spray_reset(Inst& I, hipStream_t s) {
 int inst_set_injector(Inst& I, const float* tab) {
   HIPCHK(hipMemcpy(I.tab, tab, 256 * 3 * 16, hipMemcpyHostToDevice), "upload injector table");
   I.haveTab = true;
+  I.tabSpeed = 0.f;
+  for (int t = 0; t < 256; ++t) I.tabSpeed = std::max(I.tabSpeed, std::fabs(tab[t * 4 + 3]));
   return 0;
 }
 
@@ -436,6 +438,12 @@ void fill_consts(const Inst& I, const nz_params& P, Consts& c, double tEndGrid, 
   c.resScale = P.res[0] > 0 ? (float)((double)P.res[2] * P.res[1] / P.res[0]) : 1.f;
   c.spray = (P.inj && I.haveTab) ? 1 : 0;
   c.NP = I.NP;
+  // parcel speeds stay within the injection speeds and the gas speeds (drag relaxes towards
+  // the gas, whose speed the update caps at vMax; bounces and the fan do not speed up)
+  c.depVel = std::max({P.vMax, I.tabSpeed, std::fabs(P.amb[1]) + std::fabs(P.amb[2]), 1e-6f});
+  int lg = 0;
+  while ((1LL << lg) < (long long)std::max(I.NP, 1)) ++lg;
+  c.depExp = 60 - lg;
   c.tEnd = tEndGrid > 0 ? (float)tEndGrid : 0.f;
   c.resTol = resTol > 0 ? (float)resTol : 0.f;
   c.minSteps = minSteps;
@@ -443,25 +451,46 @@ void fill_consts(const Inst& I, const nz_params& P, Consts& c, double tEndGrid, 
 }
 
 int inst_integrals(const Inst& I, const DevInst& d, const nz_probes& pr, hipStream_t s, nz_integrals* out) {
-  HIPCHK(nzk::launch_integrals(I.prec == 64, d, pr.iExit, pr.jExit, pr.iThroat, pr.iChamber, I.dint, s), "integrals");
+  nzk::IntegCtx q{};
+  q.iExit = pr.iExit; q.jExit = pr.jExit; q.iThroat = pr.iThroat; q.iChamber = pr.iChamber;
+  q.inj = d.c.inj;
+  if (pr.exact) {
+    q.pa = pr.pa;
+    for (int k = 0; k < 3; ++k) q.Rs[k] = pr.Rs[k];
+    q.RaN = pr.RaN;
+  } else {
+    q.pa = d.c.amb[3];
+    for (int k = 0; k < 3; ++k) q.Rs[k] = d.c.Rs[k];
+    q.RaN = d.c.gas[0];
+  }
+  HIPCHK(nzk::launch_integrals(I.prec == 64, d, q, I.dint + 16, I.dint, s), "integrals");
   double v[16];
-  HIPCHK(hipMemcpyAsync(v, I.dint, 11 * 8, hipMemcpyDeviceToHost, s), "integrals copy");
+  HIPCHK(hipMemcpyAsync(v, I.dint, 13 * 8, hipMemcpyDeviceToHost, s), "integrals copy");
   HIPCHK(hipStreamSynchronize(s), "integrals sync");
+  fill_integrals(v, pr, out);
+  return 0;
+}
+
+void fill_integrals(const double* v, const nz_probes& pr, nz_integrals* out) {
+  // the device sums leave dx^2 and the reference units out (simsetup.js:333-336, 248-249)
   const double dx2 = pr.dx * pr.dx;
   const double Fsi = v[0] * dx2 * P_REF;
   const double mdotSi = v[1] * dx2 * RHO_REF * U_REF;
+  const double f = dx2 * RHO_REF * U_REF;
+  const double target = (pr.exact && pr.sprayMdot > 0) ? pr.sprayMdot : 1.0;
   out->F = Fsi;
   out->mdot = mdotSi;
   out->Me = v[1] > 1e-12 ? v[2] / v[1] : 0;
   out->pe = (v[4] > 0 ? v[3] / v[4] : 0) * P_REF;
   out->Isp = mdotSi > 1e-9 ? Fsi / (mdotSi * G0) : 0;
-  out->mdotThroat = v[5] * dx2 * RHO_REF * U_REF;
+  out->mdotThroat = v[5] * f;
   out->mass = v[6];
   out->energy = v[7];
   out->Pc = v[8] * P_REF;
   out->Tc = v[9] * T_REF;
   out->residual = v[10];
-  return 0;
+  out->unburntOx = v[11] * f / std::max(target, 1e-9);
+  out->fuelVapour = v[12] * f / std::max(target, 1e-9);
 }
 
 }  // namespace nz

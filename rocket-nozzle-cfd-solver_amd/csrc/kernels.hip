@@ -57,10 +57,13 @@ __global__ void __launch_bounds__(256) k_close(const DevInst* __restrict__ insts
 // ------------------------------------------------------------------------------
 // Lagrangian parcels (FS_SPRAY_UPDATE, spray.js:40-121) with the point deposit
 // (VS_SPRAY_DEP, spray.js:131-142) into the host cell. Parcels crowd into few cells
-// near the injector, where per-parcel float atomics on the same addresses serialise;
-// so each workgroup first sums its parcels' deposits per host cell in an LDS hash table
-// (open addressing, twice as many slots as parcels, so every insert ends) and then adds
-// each distinct cell's sums to the source planes with one float atomic per channel.
+// near the injector, where per-parcel atomics on the same addresses serialise; so each
+// workgroup first sums its parcels' deposits per host cell in an LDS hash table (open
+// addressing, twice as many slots as parcels, so every insert ends) and then adds each
+// distinct cell's sums to the source planes with one atomic per channel. Every sum is of
+// 64-bit fixed-point integers (dep_scales), which add associatively: the sources, and so
+// the run, come out bit for bit the same whatever order the atomics land in, and a solve
+// gives the same bits alone and in a batch.
 constexpr int DEP_LOG = 9, DEP_SLOTS = 1 << DEP_LOG;
 static_assert(DEP_SLOTS >= 2 * 256, "deposit table: at least two slots per parcel of a workgroup");
 template <typename R>
@@ -79,13 +82,13 @@ __global__ void __launch_bounds__(256) k_spray(const DevInst* __restrict__ insts
   const int NP = c.NP;
   const int idx = ((int)blockIdx.x - I.pblk0) * 256 + threadIdx.x;
   __shared__ int depKey[DEP_SLOTS];
-  __shared__ float depVal[4][DEP_SLOTS];
+  __shared__ unsigned long long depVal[4][DEP_SLOTS];
   for (int q = threadIdx.x; q < DEP_SLOTS; q += 256) {
     depKey[q] = -1;
-    depVal[0][q] = 0.f; depVal[1][q] = 0.f; depVal[2][q] = 0.f; depVal[3][q] = 0.f;
+    depVal[0][q] = 0ull; depVal[1][q] = 0ull; depVal[2][q] = 0ull; depVal[3][q] = 0ull;
   }
   int depH = -1;   // host cell this parcel deposits into (-1: none)
-  float dep[4] = {0.f, 0.f, 0.f, 0.f};
+  unsigned long long dep[4] = {0ull, 0ull, 0ull, 0ull};
   if (idx < NP) {
     const int Nx = I.Nx, Nr = I.Nr;
     const long long Ns = I.Ns;
@@ -195,10 +198,12 @@ __global__ void __launch_bounds__(256) k_spray(const DevInst* __restrict__ insts
       gi += code_hdx(cg);
       gj += code_hdy(cg);
       depH = gj * Nx + gi;
-      dep[0] = (float)(dmo * (R(1) - bz));
-      dep[1] = (float)(dmo * bz);
-      dep[2] = (float)dpx;
-      dep[3] = (float)dpy;
+      double sc[2];
+      dep_scales(c, st->mmax, sc);
+      dep[0] = dep_fix((double)(dmo * (R(1) - bz)), sc[0], c.depExp);
+      dep[1] = dep_fix((double)(dmo * bz), sc[0], c.depExp);
+      dep[2] = dep_fix((double)dpx, sc[1], c.depExp);
+      dep[3] = dep_fix((double)dpy, sc[1], c.depExp);
     }
   }
   __syncthreads();
@@ -212,12 +217,13 @@ __global__ void __launch_bounds__(256) k_spray(const DevInst* __restrict__ insts
     for (int q = 0; q < 4; ++q) atomicAdd(&depVal[q][slot], dep[q]);
   }
   __syncthreads();
-  float* __restrict__ src = I.src;
+  unsigned long long* __restrict__ src = I.src;
   const long long Ns = I.Ns;
   for (int q = threadIdx.x; q < DEP_SLOTS; q += 256) {
     const int h = depKey[q];
     if (h < 0) continue;
-    for (int ch = 0; ch < 4; ++ch) unsafeAtomicAdd(&src[ch * Ns + h], depVal[ch][q]);
+    for (int ch = 0; ch < 4; ++ch)
+      if (depVal[ch][q]) atomicAdd(&src[ch * Ns + h], depVal[ch][q]);
   }
 }
 
@@ -323,14 +329,14 @@ __global__ void k_unpack(R* __restrict__ U, long long Ns, long long N, const flo
   for (int q = 0; q < 4; ++q) { U[q * Ns + k] = R(A[k * 4 + q]); U[(4 + q) * Ns + k] = R(B[k * 4 + q]); }
 }
 // parcels all dead (GPUSpray.reset, spray.js:242-254) and sources cleared
-__global__ void k_spray_reset(float* __restrict__ parc, int NP, float* __restrict__ src, long long Ns) {
+__global__ void k_spray_reset(float* __restrict__ parc, int NP, unsigned long long* __restrict__ src, long long Ns) {
   const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (k < NP) {
     parc[k] = -1.f; parc[NP + k] = -1.f;
     for (int q = 2; q < 8; ++q) parc[(long long)q * NP + k] = 0.f;
   }
   if (k < Ns)
-    for (int q = 0; q < 4; ++q) src[q * Ns + k] = 0.f;
+    for (int q = 0; q < 4; ++q) src[q * Ns + k] = 0ull;
 }
 
 // FP64 integrals of the current state: exit plane, throat plane, chamber probe, totals
@@ -400,59 +406,95 @@ __global__ void __launch_bounds__(256) k_derived(DevInst I, DerivedC d, const fl
   o[3] = (float)M;
 }
 
+// Integrated quantities in FP64, summed in a fixed order so that every call gives the same
+// bits. k_integrals_cells: block b sums mass and energy over fluid group hosts in its own
+// run of cells (thread t takes cells t, t + 256, ... of the run in order; block_sum folds
+// the lanes by a butterfly per wave and the four waves in order) into part[b].
+// k_integrals_close: one block folds those partials the same way, sums the exit plane
+// (exitPerformance, simsetup.js:314-337) and the throat plane (chamberPerformance,
+// 239-250), and reads the chamber probe (cellPrimSI, 338-348).
 template <typename R>
-__global__ void __launch_bounds__(256) k_integrals(DevInst I, int iExit, int jExit, int iThroat, int iChamber, double* __restrict__ out) {
-  __shared__ double red[16];
+__global__ void __launch_bounds__(256) k_integrals_cells(DevInst I, long long run, double2* __restrict__ part) {
+  __shared__ double red[8];
+  const long long N = (long long)I.Nx * I.Nr;
+  const long long k0 = (long long)blockIdx.x * run, k1 = min(k0 + run, N);
+  const R* __restrict__ U = static_cast<const R*>(I.U0);
+  const long long Ns = I.Ns;
+  double m = 0.0, e = 0.0;
+  for (long long k = k0 + threadIdx.x; k < k1; k += 256) {
+    const uint32_t cd = I.code[k];
+    if ((cd & C_SOLID) || (cd & C_IS_SLAVE)) continue;
+    const double V = (double)I.gg[k].x;
+    m += (double)U[k] * V;
+    e += (double)U[3 * Ns + k] * V;
+  }
+  const double ms = block_sum<256>(m, red);
+  __syncthreads();
+  const double es = block_sum<256>(e, red);
+  if (threadIdx.x == 0) part[blockIdx.x] = make_double2(ms, es);
+}
+
+template <typename R>
+__global__ void __launch_bounds__(256) k_integrals_close(DevInst I, IntegCtx q, const double2* __restrict__ part, int nb,
+                                                         double* __restrict__ out) {
+  __shared__ double red[8];
+  __shared__ int jmin[4];
   const int tid = threadIdx.x;
-  const R* U = static_cast<const R*>(I.U0);
+  const R* __restrict__ U = static_cast<const R*>(I.U0);
   const long long Ns = I.Ns;
   const int Nx = I.Nx, Nr = I.Nr;
-  const double pa = (double)I.c.amb[3];
-  double F = 0, md = 0, Mw = 0, pw = 0, Aa = 0, mt = 0, mass = 0, en = 0;
-  for (int j = tid; j < jExit; j += 256) {
-    const long long k = (long long)j * Nx + iExit;
+  double mass = 0.0, en = 0.0;
+  for (int b = tid; b < nb; b += 256) { mass += part[b].x; en += part[b].y; }
+  // exit plane: column iExit up to jExit, face areas of the plane iExit + 1
+  double F = 0, md = 0, Mw = 0, pw = 0, Aa = 0;
+  for (int j = tid; j < q.jExit; j += 256) {
+    const long long k = (long long)j * Nx + q.iExit;
     const double rho = fmax((double)U[k], 1e-9), u = (double)U[Ns + k] / rho, v = (double)U[2 * Ns + k] / rho;
     const double G = (double)U[6 * Ns + k], g = 1.0 + 1.0 / G;
     const double p = fmax(((double)U[3 * Ns + k] - 0.5 * rho * (u * u + v * v)) / G, 1e-9);
-    const double dA = 2.0 * M_PI * fabs((double)I.ax[(long long)j * (Nx + 1) + iExit + 1]);
-    const double cs = sqrt(g * p / rho);
+    const double dA = 2.0 * M_PI * fabs((double)I.ax[(long long)j * (Nx + 1) + q.iExit + 1]);
+    const double c = sqrt(g * p / rho);
     const double m = (double)U[4 * Ns + k] * u * dA;
-    F += (rho * u * u + p - pa) * dA;
+    F += (rho * u * u + p - q.pa) * dA;
     md += m;
-    Mw += m * sqrt(u * u + v * v) / cs;
+    Mw += m * sqrt(u * u + v * v) / c;
     pw += p * dA;
     Aa += dA;
   }
   // throat plane: fluid cells from the axis up to the first solid one
-  int jT = 0;
-  while (jT < Nr && !(I.code[(long long)jT * Nx + iThroat] & C_SOLID)) ++jT;
+  int js = Nr;
+  for (int j = tid; j < Nr; j += 256)
+    if (I.code[(long long)j * Nx + q.iThroat] & C_SOLID) js = min(js, j);
+  for (int o = 32; o > 0; o >>= 1) js = min(js, __shfl_xor(js, o, 64));
+  if ((tid & 63) == 0) jmin[tid >> 6] = js;
+  __syncthreads();
+  const int jT = min(min(jmin[0], jmin[1]), min(jmin[2], jmin[3]));
+  double mt = 0, mO = 0, mF = 0;
   for (int j = tid; j < jT; j += 256) {
-    const long long k = (long long)j * Nx + iThroat;
+    const long long k = (long long)j * Nx + q.iThroat;
     const double rho = fmax((double)U[k], 1e-9), u = (double)U[Ns + k] / rho;
-    mt += rho * u * 2.0 * M_PI * fabs((double)I.ax[(long long)j * (Nx + 1) + iThroat]);
+    const double dA = 2.0 * M_PI * fabs((double)I.ax[(long long)j * (Nx + 1) + q.iThroat]);
+    mt += rho * u * dA;
+    mO += (double)U[7 * Ns + k] * u * dA;
+    mF += (double)U[5 * Ns + k] * u * dA;
   }
-  for (long long k = tid; k < (long long)Nx * Nr; k += 256) {
-    const uint32_t cd = I.code[k];
-    if ((cd & C_SOLID) || (cd & C_IS_SLAVE)) continue;
-    const double V = (double)I.gg[k].x;
-    mass += (double)U[k] * V;
-    en += (double)U[3 * Ns + k] * V;
-  }
-  double vals[8] = {F, md, Mw, pw, Aa, mt, mass, en};
-  for (int q = 0; q < 8; ++q) {
-    const double s = block_sum<256>(vals[q], red);
+  const double vals[13] = {F, md, Mw, pw, Aa, mt, mass, en, 0.0, 0.0, 0.0, mO, mF};
+  for (int n = 0; n < 13; ++n) {
+    if (n >= 8 && n <= 10) continue;
+    const double s = block_sum<256>(vals[n], red);
     __syncthreads();
-    if (tid == 0) out[q] = s;
+    if (tid == 0) out[n] = s;
   }
   if (tid == 0) {
-    const long long k = iChamber;
+    // chamber probe at (iChamber, 0)
+    const long long k = q.iChamber;
     const double rho = fmax((double)U[k], 1e-9), u = (double)U[Ns + k] / rho, v = (double)U[2 * Ns + k] / rho;
     const double G = (double)U[6 * Ns + k];
     const double p = fmax(((double)U[3 * Ns + k] - 0.5 * rho * (u * u + v * v)) / G, 1e-9);
     const double Y = fmin(fmax((double)U[4 * Ns + k] / rho, 0.0), 1.0);
     const double Fv = fmax((double)U[5 * Ns + k] / rho, 0.0);
-    const double Ov = I.c.inj ? fmax((double)U[7 * Ns + k] / rho, 0.0) : 0.0;
-    const double Rm = (1 - Y) * 1.0 + Fv * I.c.Rs[0] + Ov * I.c.Rs[1] + fmax(Y - Fv - Ov, 0.0) * I.c.Rs[2];
+    const double Ov = q.inj ? fmax((double)U[7 * Ns + k] / rho, 0.0) : 0.0;
+    const double Rm = (1.0 - Y) * q.RaN + Fv * q.Rs[0] + Ov * q.Rs[1] + fmax(Y - Fv - Ov, 0.0) * q.Rs[2];
     out[8] = p;
     out[9] = p / (rho * Rm);
     out[10] = I.st->resLast;
@@ -580,7 +622,7 @@ hipError_t launch_pack_f64(bool fp64, const void* U, long long Ns, long long N, 
   else hipLaunchKernelGGL((k_pack_f64<float>), g, b, 0, s, static_cast<const float*>(U), Ns, N, A, B);
   return hipGetLastError();
 }
-hipError_t launch_spray_reset(float* parc, int NP, float* src, long long Ns, hipStream_t s) {
+hipError_t launch_spray_reset(float* parc, int NP, unsigned long long* src, long long Ns, hipStream_t s) {
   const long long n = NP > Ns ? NP : Ns;
   hipLaunchKernelGGL(k_spray_reset, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, parc, NP, src, Ns);
   return hipGetLastError();
@@ -592,9 +634,19 @@ hipError_t launch_derived(bool fp64, const DevInst& I, const DerivedC& d, const 
   else hipLaunchKernelGGL((k_derived<float>), g, b, 0, s, I, d, T2, out);
   return hipGetLastError();
 }
-hipError_t launch_integrals(bool fp64, const DevInst& I, int iExit, int jExit, int iThroat, int iChamber, double* out, hipStream_t s) {
-  if (fp64) hipLaunchKernelGGL((k_integrals<double>), dim3(1), dim3(256), 0, s, I, iExit, jExit, iThroat, iChamber, out);
-  else hipLaunchKernelGGL((k_integrals<float>), dim3(1), dim3(256), 0, s, I, iExit, jExit, iThroat, iChamber, out);
+hipError_t launch_integrals(bool fp64, const DevInst& I, const IntegCtx& q, double* scratch, double* out, hipStream_t s) {
+  // one block per 1024 cells, at most INTEG_BLOCKS_MAX; partials in scratch
+  const long long N = (long long)I.Nx * I.Nr;
+  const int nb = (int)std::max<long long>(1, std::min<long long>((N + 1023) / 1024, INTEG_BLOCKS_MAX));
+  const long long run = (N + nb - 1) / nb;
+  double2* part = reinterpret_cast<double2*>(scratch);
+  if (fp64) {
+    hipLaunchKernelGGL((k_integrals_cells<double>), dim3(nb), dim3(256), 0, s, I, run, part);
+    hipLaunchKernelGGL((k_integrals_close<double>), dim3(1), dim3(256), 0, s, I, q, part, nb, out);
+  } else {
+    hipLaunchKernelGGL((k_integrals_cells<float>), dim3(nb), dim3(256), 0, s, I, run, part);
+    hipLaunchKernelGGL((k_integrals_close<float>), dim3(1), dim3(256), 0, s, I, q, part, nb, out);
+  }
   return hipGetLastError();
 }
 

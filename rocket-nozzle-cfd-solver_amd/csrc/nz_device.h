@@ -55,6 +55,12 @@ struct Consts {
   float resScale;  // residual normalisation 1/rho_chamber
   int spray;       // spray stepped this step
   int NP;          // parcels
+  // fixed-point spray deposit: a release v is summed as the integer rint(v * 2^depExp / B),
+  // B = the largest spawned parcel mass (mass channels) or twice that times depVel, a bound
+  // on parcel and gas speeds (momentum channels); depExp leaves NP such terms headroom in
+  // 63 bits, so integer sums, whose order does not matter, carry every release
+  float depVel;
+  int depExp;
   // batched solves: an instance stops (flags bit 0) at t >= tEnd (grid time), after
   // minSteps steps at residual <= resTol, or after maxSteps steps; zero disables a test
   float tEnd, resTol;
@@ -78,6 +84,7 @@ struct InstState {
   uint32_t flagsSeen;  // `flags` as the last stage 2 (or close kernel) read them: stage 1, whose first
                        // workgroup writes `flags`, reads this copy instead
   uint32_t pad_;
+  double mmax;     // largest parcel mass spawned since the state was set (stage 1 raises it)
 };
 
 // The next step's stage 1 folds the tile partials of the last stage 2 into the time step
@@ -103,7 +110,8 @@ struct DevInst {
   const int* fcStart;         // per tile: first entry in fcCells; tiles + 1 entries
   const int2* fcCells;        // cells those faces reach beyond the tile's 2-cell halo
   void* exFlux;               // per extra face: 9 inviscid + 7 viscous weighted fluxes (state type)
-  float* src;                 // 4 planes of Ns: fuel vapour, ox vapour, x momentum, r momentum
+  unsigned long long* src;    // 4 planes of Ns, fixed point (Consts::depExp): fuel vapour, ox vapour,
+                              // x momentum, r momentum
   float* parc;                // 8 planes of NP: x, r, u, v, d, m, type, xi
   const float* tab;           // spray element table 256 x 3 RGBA
   InstState* st;
@@ -132,6 +140,15 @@ __device__ __forceinline__ void gst(T* base, unsigned i, T v) {
   unsigned off = i * (unsigned)sizeof(T);
   *(NZ_GLOBAL T*)((NZ_GLOBAL char*)base + off) = v;
 }
+
+// probes and parameter values of the integrated quantities (exitPerformance,
+// chamberPerformance, cellPrimSI), in double as the reference's P object holds them
+struct IntegCtx {
+  int iExit, jExit, iThroat, iChamber, inj;
+  double pa, Rs[3], RaN;
+};
+// the integrals' cell pass uses at most this many blocks (one double2 partial each)
+constexpr int INTEG_BLOCKS_MAX = 1024;
 
 // view parameters of the FS_DERIVED display fields (render.js:420-435)
 struct DerivedC {
@@ -516,5 +533,22 @@ __device__ __forceinline__ uint32_t pcg(uint32_t v) {
   return (w >> 22u) ^ w;
 }
 __device__ __forceinline__ float rnd(uint32_t a, uint32_t n) { return (float)(pcg(a + n * 7919u) & 0xffffffu) / 16777216.0f; }
+
+// Fixed-point units of the spray deposit this step: s[0] for the mass channels, s[1] for the
+// momentum channels (integer units per normalised unit)
+__device__ __forceinline__ void dep_scales(const Consts& c, double mmax, double* s) {
+  const double one = ldexp(1.0, c.depExp);
+  const double bm = fmax(mmax, 1e-300);
+  s[0] = one / bm;
+  s[1] = one / fmax(2.0 * bm * (double)c.depVel, 1e-300);
+}
+// one release in fixed point, clamped to twice its bound (a release beyond it would break the
+// bound the headroom assumes; it cannot occur for parcels spawned by this library)
+__device__ __forceinline__ unsigned long long dep_fix(double v, double scale, int depExp) {
+  const double lim = ldexp(2.0, depExp);
+  const double r = rint(v * scale);
+  const double q = r == r ? fmin(fmax(r, -lim), lim) : 0.0;  // a NaN release adds nothing
+  return (unsigned long long)(long long)q;
+}
 
 }  // namespace nzk

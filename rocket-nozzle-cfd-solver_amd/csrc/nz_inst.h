@@ -31,7 +31,9 @@ struct Inst {
   int TX = 64, TY = 4, tilesX = 0, tilesY = 0;
   void* U0 = nullptr;
   void* U1 = nullptr;
-  float *ax = nullptr, *ar = nullptr, *lt = nullptr, *jw = nullptr, *src = nullptr, *parc = nullptr, *tab = nullptr;
+  float *ax = nullptr, *ar = nullptr, *lt = nullptr, *jw = nullptr, *parc = nullptr, *tab = nullptr;
+  unsigned long long* src = nullptr;  // spray sources, fixed point
+  float tabSpeed = 0.f;               // largest injection speed of the element table
   float4* gg = nullptr;
   uint32_t* code = nullptr;
   int* exStart = nullptr;
@@ -80,5 +82,7 @@ nzk::DevInst inst_desc(const Inst& I);
 // exitPerformance / chamberPerformance sums on the device (FP64), scaled to SI;
 // `d` carries the constants in use (ambient pressure); synchronises `s`
 int inst_integrals(const Inst& I, const nzk::DevInst& d, const nz_probes& pr, hipStream_t s, nz_integrals* out);
+// the 13 device sums of the integrals kernels -> SI quantities
+void fill_integrals(const double* v, const nz_probes& pr, nz_integrals* out);
 
 }  // namespace nz

@@ -62,6 +62,8 @@ struct CutGeometry {
 };
 struct Setup {
   nz_config cfg;
+  std::vector<float> Rtab;         // the cut-cell builder's contour table (nz_cutcell_in)
+  std::vector<double> solidsFlat;  // its solids, 4 per solid (m)
   int Nx = 0, Nr = 0;
   double dx = 0, tw = 0, bodyR = 0, xBase = 0, tScale = 0;
   Contour contour;
@@ -84,10 +86,13 @@ double machFromAreaRatio(double eps, double g, bool supersonic);
 Theory nozzleTheory(double Pc, double Tc, double g, double Mw, double Dt, double De, double pa);
 Contour buildContour(const nz_config& o);
 void buildCutGeometry(const CutInput& in, CutGeometry& out);
+// the inner-wall radius (cells) every 1/8 cell, -1e9 past the exit (cutcell.js:19-27)
+std::vector<float> contourTable(const Contour& c, int Nx, double dx);
 ParamsD computeParams(const nz_config& cfg, const Contour& contour, double dx, double PcSi, double pStartSi);
 void paramsToFloat(const ParamsD& d, nz_params& P);
 int paramsToDoubles(const ParamsD& d, double* out, int n);
 Setup* buildSimSetup(const nz_config& cfg);
+void cutcellInput(const Setup& S, nz_cutcell_in& in);
 void initialState(const Setup& S, float* A, float* B);
 
 }  // namespace nz

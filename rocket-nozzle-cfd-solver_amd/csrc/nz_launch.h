@@ -25,8 +25,9 @@ hipError_t launch_pack(bool fp64, const void* U, long long Ns, int Nx, int p0, i
 hipError_t launch_unpack(bool fp64, void* U, long long Ns, long long N, const float* A, const float* B, hipStream_t s);
 hipError_t launch_step_info(const InstState* st, float* out, hipStream_t s);
 hipError_t launch_pack_f64(bool fp64, const void* U, long long Ns, long long N, double* A, double* B, hipStream_t s);
-hipError_t launch_spray_reset(float* parc, int NP, float* src, long long Ns, hipStream_t s);
+hipError_t launch_spray_reset(float* parc, int NP, unsigned long long* src, long long Ns, hipStream_t s);
 hipError_t launch_derived(bool fp64, const DevInst& I, const DerivedC& d, const float* T2, float* out, hipStream_t s);
-hipError_t launch_integrals(bool fp64, const DevInst& I, int iExit, int jExit, int iThroat, int iChamber, double* out, hipStream_t s);
+// scratch: 2 * INTEG_BLOCKS_MAX doubles; out: 13 doubles (see inst_integrals)
+hipError_t launch_integrals(bool fp64, const DevInst& I, const IntegCtx& q, double* scratch, double* out, hipStream_t s);
 
 }  // namespace nzk

@@ -547,6 +547,9 @@ __global__ void __launch_bounds__(TX* TY, (STAGE == 2 && PHYS) ? 1 : W) k_stage(
       dt = cur[0];
       if (lt_ == 0 && tid == 0) {
         st->cur[0] = cur[0]; st->cur[1] = cur[1]; st->cur[2] = cur[2]; st->cur[3] = cur[3];
+        // the parcels this step spawns weigh sprayMhat dt / 4: the bound of the deposit's
+        // fixed-point scale, which this step's spray and stage 2 read
+        if (c.spray) st->mmax = fmax(st->mmax, (double)(float)(R(c.sprayMhat) * cur[0] / R(4)));
         if (fused) {
           st->wsnap = (unsigned long long)__double_as_longlong(wF);
           st->resAcc = rF;
@@ -935,12 +938,16 @@ __global__ void __launch_bounds__(TX* TY, (STAGE == 2 && PHYS) ? 1 : W) k_stage(
 
     // ---- 5. spray gas sources (FS_SPRAY_APPLY, spray.js:158-178), consumed and cleared
     if (STAGE == 2 && c.spray && host) {
-      float* __restrict__ src = I.src;
+      unsigned long long* __restrict__ src = I.src;
       const unsigned ku = (unsigned)k;
-      const R s0 = R(gld(src, ku)), s1 = R(gld(src + Ns, ku)), s2 = R(gld(src + 2 * Ns, ku)), s3 = R(gld(src + 3 * Ns, ku));
-      const R dm = s0 + s1;
-      if (!(dm == R(0) && s2 == R(0) && s3 == R(0))) {
-        gst(src, ku, 0.f); gst(src + Ns, ku, 0.f); gst(src + 2 * Ns, ku, 0.f); gst(src + 3 * Ns, ku, 0.f);
+      const unsigned long long q0 = gld(src, ku), q1 = gld(src + Ns, ku), q2 = gld(src + 2 * Ns, ku), q3 = gld(src + 3 * Ns, ku);
+      if ((q0 | q1 | q2 | q3) != 0ull) {
+        gst(src, ku, 0ull); gst(src + Ns, ku, 0ull); gst(src + 2 * Ns, ku, 0ull); gst(src + 3 * Ns, ku, 0ull);
+        double sc[2];
+        dep_scales(c, st->mmax, sc);
+        const R s0 = R((double)(long long)q0 / sc[0]), s1 = R((double)(long long)q1 / sc[0]);
+        const R s2 = R((double)(long long)q2 / sc[1]), s3 = R((double)(long long)q3 / sc[1]);
+        const R dm = s0 + s1;
         const R invV = rc(fmax(R(gg.x), R(1e-6)));
         const R iA0 = rc(fmax(A[0], R(1e-7)));
         const R velx = A[1] * iA0, vely = A[2] * iA0;

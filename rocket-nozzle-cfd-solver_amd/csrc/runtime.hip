@@ -304,7 +304,7 @@ struct Snapshot {
   void* U0 = nullptr;
   void* U1 = nullptr;
   float* parc = nullptr;
-  float* src = nullptr;
+  void* src = nullptr;
   void* part = nullptr;  // the last step's tile partials, folded by the next step
   InstState st{};
   ~Snapshot() {
@@ -317,7 +317,7 @@ struct Snapshot {
 };
 int snap_take(nz_solver* s, Snapshot& z) {
   const nz::Inst& I = s->I;
-  const size_t us = (size_t)I.Ns * 8 * I.rsz, ps = (size_t)I.NP * 8 * 4, ss = (size_t)I.Ns * 16;
+  const size_t us = (size_t)I.Ns * 8 * I.rsz, ps = (size_t)I.NP * 8 * 4, ss = (size_t)I.Ns * 32;
   HIPCHK(hipMalloc(&z.U0, us), "snapshot");
   HIPCHK(hipMalloc(&z.U1, us), "snapshot");
   HIPCHK(hipMalloc(&z.parc, ps), "snapshot");
@@ -334,7 +334,7 @@ int snap_take(nz_solver* s, Snapshot& z) {
 }
 int snap_restore(nz_solver* s, const Snapshot& z) {
   const nz::Inst& I = s->I;
-  const size_t us = (size_t)I.Ns * 8 * I.rsz, ps = (size_t)I.NP * 8 * 4, ss = (size_t)I.Ns * 16;
+  const size_t us = (size_t)I.Ns * 8 * I.rsz, ps = (size_t)I.NP * 8 * 4, ss = (size_t)I.Ns * 32;
   HIPCHK(hipMemcpyAsync(I.U0, z.U0, us, hipMemcpyDeviceToDevice, s->stream), "restore");
   HIPCHK(hipMemcpyAsync(I.U1, z.U1, us, hipMemcpyDeviceToDevice, s->stream), "restore");
   HIPCHK(hipMemcpyAsync(I.parc, z.parc, ps, hipMemcpyDeviceToDevice, s->stream), "restore");

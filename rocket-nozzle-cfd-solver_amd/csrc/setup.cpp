@@ -285,6 +285,16 @@ static std::vector<Solid> chamberSolids(const nz_config& cfg, const Contour& c, 
 }
 
 // ---- cut-cell geometry (cutcell.js:14-186)
+std::vector<float> contourTable(const Contour& contour, int Nx, double dx) {
+  const int SUB = 8, nTab = Nx * SUB + 1;
+  std::vector<float> Rtab(nTab);
+  for (int k = 0; k < nTab; ++k) {
+    const double X = ((double)k / SUB) * dx;
+    Rtab[k] = f32(X <= contour.xe ? contour.radiusAt(X) / dx : -1e9);
+  }
+  return Rtab;
+}
+
 void buildCutGeometry(const CutInput& o, CutGeometry& G) {
   const int Nx = o.Nx, Nr = o.Nr;
   const double dx = o.dx;
@@ -293,11 +303,7 @@ void buildCutGeometry(const CutInput& o, CutGeometry& G) {
   const double xeC = contour.xe / dx, twC = o.tw / dx, xBaseC = o.xBase / dx, bodyC = o.bodyR / dx;
   const int SUB = 8;
   const int nTab = Nx * SUB + 1;
-  std::vector<float> Rtab(nTab);
-  for (int k = 0; k < nTab; ++k) {
-    const double X = ((double)k / SUB) * dx;
-    Rtab[k] = f32(X <= contour.xe ? contour.radiusAt(X) / dx : -1e9);
-  }
+  const std::vector<float> Rtab = contourTable(contour, Nx, dx);
   auto RinAt = [&](double xc) -> double {
     if (xc > xeC) return -1e9;
     const double f = std::min(std::max(xc * SUB, 0.0), (double)(nTab - 1));
@@ -766,6 +772,22 @@ static std::vector<float> buildInjectorTable(const nz_config& cfg, const Contour
   return tab;
 }
 
+// the cut-cell builder's input for this setup (pointers into the setup)
+void cutcellInput(const Setup& S, nz_cutcell_in& in) {
+  std::memset(&in, 0, sizeof(in));
+  in.Nx = S.Nx; in.Nr = S.Nr; in.dx = S.dx; in.tw = S.tw; in.xBase = S.xBase; in.bodyR = S.bodyR;
+  in.base = S.cfg.base ? 1 : 0;
+  in.stair = S.cfg.wallModel == NZ_WALL_STAIR ? 1 : 0;
+  in.Rtab = S.Rtab.data();
+  in.npts = (int32_t)S.contour.xs.size();
+  in.xs = S.contour.xs.data();
+  in.rs = S.contour.rs.data();
+  in.Rc = S.contour.Rc; in.Re = S.contour.Re; in.xe = S.contour.xe;
+  in.cavD = S.contour.cavD; in.cavW = S.contour.cavW;
+  in.nsolids = (int32_t)(S.solidsFlat.size() / 4);
+  in.solids = S.solidsFlat.empty() ? nullptr : S.solidsFlat.data();
+}
+
 // ---- buildSimSetup (simsetup.js:7-126)
 Setup* buildSimSetup(const nz_config& cfg_in) {
   Setup* S = new Setup();
@@ -810,7 +832,31 @@ Setup* buildSimSetup(const nz_config& cfg_in) {
   CutInput ci;
   ci.Nx = Nx; ci.Nr = Nr; ci.dx = dx; ci.contour = &contour; ci.tw = tw; ci.base = cfg.base != 0;
   ci.xBase = xBase; ci.bodyR = bodyR; ci.stair = cfg.wallModel == NZ_WALL_STAIR; ci.solids = hw;
-  buildCutGeometry(ci, S->geo);
+  S->cfg = cfg; S->Nx = Nx; S->Nr = Nr; S->dx = dx; S->tw = tw; S->bodyR = bodyR; S->xBase = xBase;
+  S->contour = contour;
+  S->Rtab = contourTable(contour, Nx, dx);
+  S->solidsFlat.clear();
+  for (const Solid& q : hw) S->solidsFlat.insert(S->solidsFlat.end(), {q.x0, q.x1, q.r0, q.r1});
+  if (cfg.deviceGeometry) {
+    // the same geometry, built by the GPU kernels (cutcell.hip)
+    CutGeometry& G = S->geo;
+    const size_t N = (size_t)Nx * Nr, NL = (size_t)(Nx + 1) * (Nr + 1);
+    G.Nx = Nx; G.Nr = Nr;
+    G.mask.assign(N, 0); G.geoA.assign(NL * 4, 0.f); G.geoG.assign(N * 4, 0.f); G.geoM.assign(N * 4, 0.f);
+    G.kappa.assign(N, 0.f); G.Ax.assign(NL, 0.f); G.Ar.assign(NL, 0.f);
+    nz_cutcell_in in;
+    cutcellInput(*S, in);
+    int32_t counts[2] = {0, 0};
+    if (nz_cutcell_build(&in, G.mask.data(), G.geoA.data(), G.geoG.data(), G.geoM.data(), G.kappa.data(), G.Ax.data(),
+                         G.Ar.data(), counts) != 0) {
+      delete S;
+      return nullptr;
+    }
+    G.nCut = counts[0];
+    G.nMerged = counts[1];
+  } else {
+    buildCutGeometry(ci, S->geo);
+  }
   CutGeometry& geo = S->geo;
   const std::vector<uint8_t>& mask = geo.mask;
   // turbulence length scale and boundary-layer band flag (simsetup.js:51-63)
@@ -1021,6 +1067,7 @@ int nz_setup_build(const nz_config* cfg, nz_setup** out) {
     return nz_fail(-1, "nz_setup_build: invalid configuration");
   try {
     *out = reinterpret_cast<nz_setup*>(buildSimSetup(*cfg));
+    if (!*out) return -10;   // the device geometry build failed; nz_last_error() says why
   } catch (const std::exception& e) {
     return nz_fail(-3, std::string("nz_setup_build: ") + e.what());
   }
@@ -1090,6 +1137,29 @@ int nz_setup_get_injector(const nz_setup* sp, float* tab) {
   const Setup& S = *reinterpret_cast<const Setup*>(sp);
   if (!S.injOn) return -1;
   if (tab) std::memcpy(tab, S.injTab.data(), S.injTab.size() * 4);
+  return 0;
+}
+
+int nz_setup_get_cutcell_input(const nz_setup* sp, nz_cutcell_in* out) {
+  if (!sp || !out) return nz_fail(-1, "nz_setup_get_cutcell_input: null argument");
+  nz::cutcellInput(*reinterpret_cast<const Setup*>(sp), *out);
+  return 0;
+}
+
+int nz_setup_get_probes(const nz_setup* sp, nz_probes* out) {
+  if (!sp || !out) return nz_fail(-1, "nz_setup_get_probes: null argument");
+  const Setup& S = *reinterpret_cast<const Setup*>(sp);
+  std::memset(out, 0, sizeof(*out));
+  out->iExit = S.iExit;
+  out->jExit = S.jExit;
+  out->iThroat = S.iThroat;
+  out->iChamber = S.iChamber;
+  out->dx = S.dx;
+  out->exact = 1;
+  out->pa = S.P.amb[3];
+  for (int k = 0; k < 3; ++k) out->Rs[k] = S.P.Rs[k];
+  out->RaN = 1.0;   // computeParams' RaN (simsetup.js:309)
+  out->sprayMdot = S.P.inj ? S.P.sprayMdot : 0.0;
   return 0;
 }
 

@@ -58,14 +58,16 @@ const ch = solver.readCell(setup.iChamber, 0);
 const all = solver.readAll();
 let sumA = 0, sumB = 0, finite = true;
 for (let k = 0; k < all.A.length; k++) { sumA += all.A[k]; sumB += all.B[k]; if (!isFinite(all.A[k])) finite = false; }
-// exitPerformance (simsetup.js:314-337) on the read-back column
+// exitPerformance (simsetup.js:314-337) on the read-back column, with the ambient pressure
+// at the double precision of the reference's P (the setup's float uniforms round it)
+const pa = setup.probes ? setup.probes.pa : P.amb[3];
 let F = 0;
 for (let j = 0; j < setup.jExit; j++) {
   const k = j * 4, rho = Math.max(col.A[k], 1e-9), u = col.A[k + 1] / rho, v = col.A[k + 2] / rho;
   const G = col.B[k + 2];
   const p = Math.max((col.A[k + 3] - 0.5 * rho * (u * u + v * v)) / G, 1e-9);
   const dA = 2 * Math.PI * Math.abs(setup.geo.Ax[j * setup.geo.W1 + setup.iExit + 1]) * setup.dx * setup.dx;
-  F += (rho * u * u + p - P.amb[3]) * dA;
+  F += (rho * u * u + p - pa) * dA;
 }
 const integ = solver.integrals(P);
 // render.js derive(V) for the Mach field (app.js V() fills the view), and the HUD

@@ -84,7 +84,19 @@ class GPUSolver {
   readAll() { return { A: this._readRect(this.state.A, 0, 0, this.Nx, this.Nr), B: this._readRect(this.state.B, 0, 0, this.Nx, this.Nr) }; }
   // build-defined extras (no counterpart in the reference)
   sync() { addon.sync(this._h); }
-  setProbes(setup) { addon.setProbes(this._h, setup.iExit, setup.jExit, setup.iThroat, setup.iChamber, setup.dx); }
+  // the probes of exitPerformance / chamberPerformance / cellPrimSI, with the setup's P values
+  // in double (simsetup.js:239-348) when it carries them
+  setProbes(setup) {
+    const P = setup.P;
+    const q = setup.probes || (P && P.amb && P.Rs
+      ? { pa: P.amb[3], Rs: P.Rs, RaN: P.RaN === undefined ? 1 : P.RaN, sprayMdot: P.spray ? P.spray.mdot : 0 } : null);
+    if (q) {
+      addon.setProbes(this._h, setup.iExit, setup.jExit, setup.iThroat, setup.iChamber, setup.dx, q.pa,
+        q.Rs[0], q.Rs[1], q.Rs[2], q.RaN, q.sprayMdot);
+    } else {
+      addon.setProbes(this._h, setup.iExit, setup.jExit, setup.iThroat, setup.iChamber, setup.dx);
+    }
+  }
   integrals(P) { return addon.integrals(this._h, P); }
   // render.js derive(V, second) (render.js:420-435): the derived field's Nx x Nr x
   // (t, Z, Tn, M) values, returned instead of drawn into derivedFbo
@@ -98,7 +110,7 @@ class GPUSolver {
 class GPUSpray {
   constructor(solver) {
     this.s = solver;
-    this.blendOk = true;   // float atomics on the device stand in for EXT_float_blend (spray.js:215)
+    this.blendOk = true;   // fixed-point integer atomics on the device stand in for EXT_float_blend (spray.js:215)
     this.ready = false;
   }
   resize() { this.ready = true; this.reset(); }

@@ -28,8 +28,8 @@ napi_value throw_rc(napi_env env, int rc, const char* what) {
 }
 
 struct Args {
-  size_t n = 8;
-  napi_value v[8];
+  size_t n = 16;
+  napi_value v[16];
   napi_value self;
 };
 bool get_args(napi_env env, napi_callback_info info, Args& a) {
@@ -386,7 +386,20 @@ napi_value SprayStats(napi_env env, napi_callback_info info) {
 napi_value Probes(napi_env env, napi_callback_info info) {
   Args a;
   if (!get_args(env, info, a) || a.n < 6) return throw_rc(env, -1, "setProbes");
-  nz_probes p{(int32_t)num(env, a.v[1]), (int32_t)num(env, a.v[2]), (int32_t)num(env, a.v[3]), (int32_t)num(env, a.v[4]), num(env, a.v[5])};
+  nz_probes p{};
+  p.iExit = (int32_t)num(env, a.v[1]);
+  p.jExit = (int32_t)num(env, a.v[2]);
+  p.iThroat = (int32_t)num(env, a.v[3]);
+  p.iChamber = (int32_t)num(env, a.v[4]);
+  p.dx = num(env, a.v[5]);
+  // (pa, Rs0, Rs1, Rs2, RaN, sprayMdot): the reference's P values in double
+  if (a.n >= 12) {
+    p.exact = 1;
+    p.pa = num(env, a.v[6]);
+    for (int k = 0; k < 3; ++k) p.Rs[k] = num(env, a.v[7 + k]);
+    p.RaN = num(env, a.v[10]);
+    p.sprayMdot = num(env, a.v[11]);
+  }
   if (int rc = nz_set_probes(solver_of(env, a.v[0]), &p)) return throw_rc(env, rc, "setProbes");
   return nullptr;
 }
@@ -410,6 +423,8 @@ napi_value Integrals(napi_env env, napi_callback_info info) {
   set(env, o, "mass", mk_num(env, r.mass));
   set(env, o, "energy", mk_num(env, r.energy));
   set(env, o, "residual", mk_num(env, r.residual));
+  set(env, o, "unburntOx", mk_num(env, r.unburntOx));
+  set(env, o, "fuelVapour", mk_num(env, r.fuelVapour));
   return o;
 }
 // render.js derive(V) (render.js:420-435): V carries fieldIndex, range, RaN, Rs, chemK,
@@ -536,6 +551,19 @@ napi_value BuildSetup(napi_env env, napi_callback_info info) {
   set(env, init, "B", new_f32(env, B.data(), B.size()));
   set(env, out, "init", init);
   set(env, out, "P", params_to_js(env, P, nullptr));
+  // the double-precision P values the integrated quantities use (as the reference's P)
+  nz_probes pr{};
+  nz_setup_get_probes(s, &pr);
+  napi_value pq;
+  napi_create_object(env, &pq);
+  set(env, pq, "pa", mk_num(env, pr.pa));
+  napi_value rs;
+  napi_create_array_with_length(env, 3, &rs);
+  for (uint32_t k = 0; k < 3; ++k) napi_set_element(env, rs, k, mk_num(env, pr.Rs[k]));
+  set(env, pq, "Rs", rs);
+  set(env, pq, "RaN", mk_num(env, pr.RaN));
+  set(env, pq, "sprayMdot", mk_num(env, pr.sprayMdot));
+  set(env, out, "probes", pq);
   if (injector) {
     napi_value inj;
     napi_create_object(env, &inj);

@@ -58,6 +58,17 @@ class NzConfig(C.Structure):
         ("film", C.c_double), ("baffles", C.c_double), ("baffleH", C.c_double), ("cavD", C.c_double), ("cavW", C.c_double),
         ("dx", C.c_double), ("hllBlend", C.c_double),
         ("convAngle", C.c_double), ("ambP", C.c_double), ("ambT", C.c_double), ("turbulence", C.c_int32),
+        ("deviceGeometry", C.c_int32),
+    ]
+
+
+class NzCutcellIn(C.Structure):
+    _fields_ = [
+        ("Nx", C.c_int32), ("Nr", C.c_int32), ("dx", C.c_double), ("tw", C.c_double), ("xBase", C.c_double),
+        ("bodyR", C.c_double), ("base", C.c_int32), ("stair", C.c_int32), ("Rtab", C.POINTER(C.c_float)),
+        ("npts", C.c_int32), ("xs", C.POINTER(C.c_double)), ("rs", C.POINTER(C.c_double)),
+        ("Rc", C.c_double), ("Re", C.c_double), ("xe", C.c_double), ("cavD", C.c_double), ("cavW", C.c_double),
+        ("nsolids", C.c_int32), ("solids", C.POINTER(C.c_double)),
     ]
 
 
@@ -85,12 +96,14 @@ class NzSprayStats(C.Structure):
 
 class NzProbes(C.Structure):
     _fields_ = [("iExit", C.c_int32), ("jExit", C.c_int32), ("iThroat", C.c_int32), ("iChamber", C.c_int32),
-                ("dx", C.c_double)]
+                ("dx", C.c_double), ("exact", C.c_int32), ("pa", C.c_double), ("Rs", C.c_double * 3),
+                ("RaN", C.c_double), ("sprayMdot", C.c_double)]
 
 
 class NzIntegrals(C.Structure):
     _fields_ = [(n, C.c_double) for n in
-                ("F", "mdot", "Me", "pe", "Isp", "mdotThroat", "Pc", "Tc", "mass", "energy", "residual")]
+                ("F", "mdot", "Me", "pe", "Isp", "mdotThroat", "Pc", "Tc", "mass", "energy", "residual",
+                 "unburntOx", "fuelVapour")]
 
 
 class NzDerivedParams(C.Structure):
@@ -195,6 +208,9 @@ SIGNATURES = {
     "nz_setup_get_geometry": (C.c_int, [vp, u8p, fp, fp, fp, fp, fp, fp, fp]),
     "nz_setup_initial_state": (C.c_int, [vp, fp, fp]),
     "nz_setup_get_injector": (C.c_int, [vp, fp]),
+    "nz_setup_get_probes": (C.c_int, [vp, P(NzProbes)]),
+    "nz_setup_get_cutcell_input": (C.c_int, [vp, P(NzCutcellIn)]),
+    "nz_cutcell_build": (C.c_int, [P(NzCutcellIn), u8p, fp, fp, fp, fp, fp, fp, P(C.c_int32)]),
     "nz_setup_radius_at": (C.c_double, [vp, C.c_double]),
     "nz_wall_separation": (C.c_int, [vp, fp, C.c_double, P(NzSeparation)]),
     "nz_summerfield_point": (C.c_int, [vp, C.c_double, C.c_double, C.c_double, dp]),

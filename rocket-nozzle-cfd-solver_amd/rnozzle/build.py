@@ -15,7 +15,10 @@ INCLUDE = os.path.join(REPO, "include")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
-HIP_SOURCES = ["kernels.hip", "instance.hip", "runtime.hip", "batch.hip"]
+HIP_SOURCES = ["kernels.hip", "instance.hip", "runtime.hip", "batch.hip", "cutcell.hip"]
+# per-file flags: the cut-cell geometry must round exactly as the reference's JavaScript
+# (FP64 with float32 stores), so no multiply-add contraction there
+FILE_FLAGS = {"cutcell.hip": ["-ffp-contract=off"]}
 CPP_SOURCES = ["setup.cpp", "errors.cpp"]
 
 # device code: no packed-FP32 instructions (they are avoided on this platform); small
@@ -52,7 +55,7 @@ def build_lib(jobs=8):
     for src in HIP_SOURCES:
         obj = os.path.join(obj_dir, src + ".o")
         cmd = [HIPCC, f"--offload-arch={ARCH}", "-c", *common, *DEVICE_FLAGS, "-munsafe-fp-atomics",
-               os.path.join(CSRC, src), "-o", obj]
+               *FILE_FLAGS.get(src, []), os.path.join(CSRC, src), "-o", obj]
         procs.append((cmd, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)))
         objs.append(obj)
     # one object per tile shape of the stage kernels

@@ -103,7 +103,10 @@ class SimSetup:
         return lib().nz_setup_radius_at(self._h, x)
 
     def probes(self):
-        return abi.NzProbes(self.iExit, self.jExit, self.iThroat, self.iChamber, self.dx)
+        """Probes with the parameter block's double-precision values (nz_setup_get_probes)."""
+        pr = abi.NzProbes()
+        check(lib().nz_setup_get_probes(self._h, C.byref(pr)), "nz_setup_get_probes")
+        return pr
 
     def wall_separation(self, row, paSi):
         """wallSeparation(setup, row, paSi) (simsetup.js:352-373) on the near-wall row

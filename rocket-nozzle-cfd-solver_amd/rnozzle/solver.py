@@ -26,7 +26,7 @@ class GPUSpray:
 
     def __init__(self, solver):
         self.s = solver
-        self.blendOk = True      # float atomics replace EXT_float_blend (spray.js:215)
+        self.blendOk = True      # fixed-point integer atomics replace EXT_float_blend (spray.js:215)
         self.ready = True
         self.n = 65536
 
@@ -251,8 +251,10 @@ class GPUSolver:
         self._async = None
 
     # ---- build-defined extras
-    def setProbes(self, iExit, jExit, iThroat, iChamber, dx):
-        pr = abi.NzProbes(iExit, jExit, iThroat, iChamber, dx)
+    def setProbes(self, iExit, jExit=None, iThroat=None, iChamber=None, dx=None):
+        """Probes of the integrated quantities: an NzProbes (SimSetup.probes(), with the
+        parameter block's double-precision values), or the four indices and dx."""
+        pr = iExit if isinstance(iExit, abi.NzProbes) else abi.NzProbes(iExit, jExit, iThroat, iChamber, dx)
         check(lib().nz_set_probes(self._h, C.byref(pr)), "setProbes")
 
     def derived(self, field, rng, P, Tc, Ta, schlGain, dx):
@@ -288,5 +290,5 @@ def load_setup(solver, setup):
         solver.spray.setInjector({"tab": setup.injector})
     A, B = setup.initial_state()
     solver.setState(A, B)
-    solver.setProbes(setup.iExit, setup.jExit, setup.iThroat, setup.iChamber, setup.dx)
+    solver.setProbes(setup.probes())
     return A, B

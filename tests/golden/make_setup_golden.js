@@ -1,6 +1,8 @@
 // make_setup_golden.js — writes input/output vectors of the reference's host-side
 // setup functions (buildSimSetup, computeParams, buildCutGeometry, initialState,
-// buildInjectorTable, nozzleTheory, atmosphere) for a set of configurations.
+// buildInjectorTable, nozzleTheory, atmosphere) for a set of configurations, and of the
+// integrated quantities (exitPerformance, chamberPerformance, cellPrimSI) on a seeded
+// synthetic state of each.
 //
 // Runs in the development container only, under node, on the reference's own
 // source files read from the directory given as argv[2]. Nothing from the
@@ -20,7 +22,7 @@ fs.mkdirSync(outDir, { recursive: true });
 // the host-side ones are loaded here the same way, into one context
 const files = ['physics.js', 'cutcell.js', 'simsetup.js'];
 let code = files.map((f) => fs.readFileSync(path.join(srcDir, f), 'utf8')).join('\n;\n');
-code += '\n;this.__api = { buildSimSetup, computeParams, buildContour, nozzleTheory, atmosphere, exitPerformance };\n';
+code += '\n;this.__api = { buildSimSetup, computeParams, buildContour, nozzleTheory, atmosphere, exitPerformance, chamberPerformance, cellPrimSI };\n';
 const ctx = vm.createContext({ Math, Float32Array, Float64Array, Uint8Array, Int32Array, Array, Object, JSON, Infinity, NaN, isFinite });
 vm.runInContext(code, ctx, { filename: 'reference-host.js' });
 const api = ctx.__api;
@@ -76,7 +78,31 @@ function dump(name, arr) {
   const buf = Buffer.from(arr.buffer, arr.byteOffset, arr.byteLength);
   fs.writeFileSync(path.join(outDir, name), buf);
 }
+// a seeded flow over the setup's grid (32-bit LCG, Numerical Recipes constants): density
+// and pressure jittered about the initial state, a streamwise velocity of up to 1.3
+// U_REF, radial velocity, and exhaust / fuel / oxidiser fractions, stored as Float32 the
+// way the solver's textures hold them
+function syntheticState(s, init, seed) {
+  let x = seed >>> 0;
+  const rnd = () => { x = (Math.imul(x, 1664525) + 1013904223) >>> 0; return x / 4294967296; };
+  const N = s.Nx * s.Nr, A = new Float32Array(N * 4), B = new Float32Array(N * 4);
+  for (let k = 0; k < N; k++) {
+    const q = k * 4, G = init.B[q + 2] > 0.5 ? init.B[q + 2] : 2.5;
+    const r0 = Math.max(init.A[q], 1e-3), p0 = Math.max((init.A[q + 3] - 0.5 * (init.A[q + 1] ** 2 + init.A[q + 2] ** 2) / r0) / G, 1e-3);
+    const rho = r0 * (0.8 + 0.4 * rnd()), u = 1.5 * rnd() - 0.2, v = 0.3 * (rnd() - 0.5), p = p0 * (0.8 + 0.4 * rnd());
+    const Z = rnd(), F = Z * rnd(), O = (1 - Z) * rnd();
+    A[q] = rho; A[q + 1] = rho * u; A[q + 2] = rho * v; A[q + 3] = p * G + 0.5 * rho * (u * u + v * v);
+    B[q] = rho * Z; B[q + 1] = rho * F; B[q + 2] = G; B[q + 3] = rho * O;
+  }
+  return { A, B };
+}
+function column(s, st, i) {
+  const A = new Float32Array(s.Nr * 4), B = new Float32Array(s.Nr * 4);
+  for (let j = 0; j < s.Nr; j++) for (let c = 0; c < 4; c++) { A[j * 4 + c] = st.A[(j * s.Nx + i) * 4 + c]; B[j * 4 + c] = st.B[(j * s.Nx + i) * 4 + c]; }
+  return { A, B };
+}
 const index = {};
+let caseSeed = 20240501;
 for (const [name, cfg] of Object.entries(cases)) {
   const s = api.buildSimSetup(JSON.parse(JSON.stringify(cfg)));
   const init = s.initialState();
@@ -96,13 +122,28 @@ for (const [name, cfg] of Object.entries(cases)) {
   const xs = []; for (let k = 0; k <= 400; k++) xs.push(s.contour.xe * 1.1 * k / 400 - 0.01);
   dump(`${name}.radius.f64`, new Float64Array(xs.map((x) => s.contour.radiusAt(x))));
   dump(`${name}.radius_x.f64`, new Float64Array(xs));
+  // integrated quantities of a seeded flow (app.js:534-539, 597)
+  // (only what the three functions read is kept: the exit and throat columns, the chamber cell)
+  const syn = syntheticState(s, init, caseSeed++);
+  const colE = column(s, syn, s.iExit), colT = column(s, syn, s.iThroat);
+  const kc = s.iChamber * 4;
+  const cellA = syn.A.slice(kc, kc + 4), cellB = syn.B.slice(kc, kc + 4);
+  dump(`${name}.exitA.f32`, colE.A); dump(`${name}.exitB.f32`, colE.B);
+  dump(`${name}.throatA.f32`, colT.A); dump(`${name}.throatB.f32`, colT.B);
+  dump(`${name}.cellA.f32`, cellA); dump(`${name}.cellB.f32`, cellB);
+  const integ = {
+    exit: api.exitPerformance(s, colE),
+    chamber: api.chamberPerformance(s, colT),
+    cell: api.cellPrimSI(s, Array.from(cellA), Array.from(cellB)),
+  };
   const th = s.theory;
   index[name] = {
     cfg, Nx: s.Nx, Nr: s.Nr, dx: s.dx, tw: s.tw, bodyR: s.bodyR, xBase: s.xBase, tScale: s.tScale,
     iExit: s.iExit, jExit: s.jExit, iChamber: s.iChamber, iThroat: s.iThroat, nCut: s.geo.nCut, nMerged: s.geo.nMerged,
     contour: { Rt: s.contour.Rt, Re: s.contour.Re, Rc: s.contour.Rc, Lc: s.contour.Lc, xt: s.contour.xt, xe: s.contour.xe, eps: s.contour.eps, thn: s.contour.thn, the: s.contour.the },
     atm: s.atm, theory: { Me: th.Me, pe: th.pe, mdot: th.mdot, F: th.F, Isp: th.Isp, cstar: th.cstar, c0: th.c0, uLim: th.uLim },
-    injector: !!s.injector, P2args: [0.8 * cfg.Pc, 0.5 * s.atm.p],
+    injector: !!s.injector, P2args: [0.8 * cfg.Pc, 0.5 * s.atm.p], integ,
+    Pd: { pa: s.P.amb[3], Rs: s.P.Rs, RaN: s.P.RaN, inj: s.P.inj, sprayMdot: s.P.spray ? s.P.spray.mdot : 0 },
   };
 }
 // atmosphere and nozzle theory at a sweep of inputs

@@ -36,7 +36,7 @@ def test_struct_sizes():
     # field-by-field mirrors of the header structs
     assert C.sizeof(abi.NzParams) == 4 * abi.N_PARAMS_F64 == 252
     assert C.sizeof(abi.NzReadReq) == 20
-    assert C.sizeof(abi.NzIntegrals) == 8 * 11
+    assert C.sizeof(abi.NzIntegrals) == 8 * 13
 
 
 def test_last_error_and_null_handling():
@@ -64,7 +64,8 @@ def test_struct_layout_matches_header(tmp_path):
     mirrors = {"nz_params": abi.NzParams, "nz_read_req": abi.NzReadReq, "nz_spray_stats_t": abi.NzSprayStats,
                "nz_probes": abi.NzProbes, "nz_integrals": abi.NzIntegrals, "nz_config": abi.NzConfig,
                "nz_setup_info": abi.NzSetupInfo, "nz_batch_job": abi.NzBatchJob, "nz_batch_result": abi.NzBatchResult,
-               "nz_derived_params": abi.NzDerivedParams, "nz_separation": abi.NzSeparation}
+               "nz_derived_params": abi.NzDerivedParams, "nz_separation": abi.NzSeparation,
+               "nz_cutcell_in": abi.NzCutcellIn}
     lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "nozzle.h"', "int main(void) {"]
     for cname, py in mirrors.items():
         lines.append(f'  printf("{cname} sizeof %zu\\n", sizeof({cname}));')
