@@ -391,6 +391,14 @@ int nz_setup_get_cutcell_input(const nz_setup* s, nz_cutcell_in* out);
  * Allocates its device buffers for the call and frees them. */
 int nz_cutcell_build(const nz_cutcell_in* in, uint8_t* mask, float* geoA, float* geoG, float* geoM, float* kappa,
                      float* Ax, float* Ar, int32_t* counts);
+/* nz_cutcell_build without any device allocation. No counterpart in the reference: for
+ * callers that cannot allocate (code under graph capture, serving loops admitting new
+ * geometries). Inputs, intermediates and outputs use one 36 MiB device array per GPU, held
+ * until the process ends: grids beyond about 1024 x 384 cells (or more than 256 solids) are
+ * refused with an error. Runs on the null stream and waits for it. Calls that overlap on
+ * one GPU race on that array. Same results as nz_cutcell_build, bit for bit. */
+int nz_cutcell_build_noalloc(const nz_cutcell_in* in, uint8_t* mask, float* geoA, float* geoG, float* geoM,
+                             float* kappa, float* Ax, float* Ar, int32_t* counts);
 /* HUD read-outs on the device's near-wall row (nz_draw_wall_row + NZ_FIELD_WALLROW:
  * Nx x (u, v, p, c), normalised). wallSeparation (simsetup.js:352-373): returns 1
  * and fills *out when the bell has separated, 0 while attached. */

@@ -13,6 +13,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -489,5 +490,85 @@ extern "C" int nz_cutcell_build(const nz_cutcell_in* in, uint8_t* mask, float* g
 done:
   if (base) (void)hipFree(base);
   if (s) (void)hipStreamDestroy(s);
+  return rc;
+}
+
+// ------------------------------------------------------------------------------------
+// The same build without any allocation, for callers that cannot allocate device memory
+// (code under graph capture, serving loops that admit new geometries on the fly). Inputs,
+// intermediates and outputs live in one file-scope device array per GPU, held until the
+// process ends; grids whose layout does not fit it are refused. Every call on a GPU shares
+// that GPU's array: calls that overlap on one GPU race, and each call clears what must start
+// at zero (the counters) itself.
+namespace nzc {
+constexpr size_t kNoAllocScratchBytes = (size_t)36 << 20;   // 36 MiB: 1024 x 384 cells fit
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+// the array's device address, looked up once per device (the host-side name of a
+// __device__ array is a stand-in, not a device address)
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratchMu;
+static void* g_scratchPtr[kMaxDevices] = {};
+
+int scratch_address(void** out) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices)
+    return nz_fail(-10, "nz_cutcell_build_noalloc: no current HIP device");
+  std::lock_guard<std::mutex> lk(g_scratchMu);
+  if (!g_scratchPtr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return nz_fail(-10, std::string("nz_cutcell_build_noalloc: scratch address: ") + hipGetErrorString(e));
+    g_scratchPtr[dev] = p;
+  }
+  *out = g_scratchPtr[dev];
+  return 0;
+}
+}  // namespace nzc
+
+extern "C" int nz_cutcell_build_noalloc(const nz_cutcell_in* in, uint8_t* mask, float* geoA, float* geoG, float* geoM,
+                                        float* kappa, float* Ax, float* Ar, int32_t* counts) {
+  using namespace nzc;
+  CutParams p{};
+  Sizes z{};
+  if (int r = check_input(in, "nz_cutcell_build_noalloc", p, z)) return r;
+  if (!mask || !geoA || !geoG || !geoM) return nz_fail(-1, "nz_cutcell_build_noalloc: null output");
+  if (in->nsolids > 256) return nz_fail(-1, "nz_cutcell_build_noalloc: more than 256 solids");
+  const size_t need = layout_bytes(z, true);
+  if (need > kNoAllocScratchBytes)
+    return nz_fail(-8, "nz_cutcell_build_noalloc: a " + std::to_string(in->Nx) + " x " + std::to_string(in->Nr) +
+                           " grid needs " + std::to_string(need) + " bytes of device scratch, more than its " +
+                           std::to_string(kNoAllocScratchBytes) + " (at most about 1024 x 384 cells); use nz_cutcell_build");
+  void* base = nullptr;
+  if (int r = scratch_address(&base)) return r;
+  int rc = 0;
+  CutBufs b{};
+  float* dR = nullptr;
+  double *dxs = nullptr, *drs = nullptr, *dsol = nullptr;
+  carve(static_cast<char*>(base), z, true, b, &dR, &dxs, &drs, &dsol);
+  // the solids in cells, staged in host memory the caller's call owns (no allocation: at most 256)
+  double sol[256 * 4];
+  for (int s = 0; s < in->nsolids * 4; ++s) sol[s] = in->solids[s] / in->dx;
+  const size_t nsolBytes = (size_t)std::max(in->nsolids, 1) * 32;
+  if (in->nsolids == 0) std::memset(sol, 0, 32);
+  unsigned int cnt[4] = {0, 0, 0, 0};
+  const hipStream_t s = nullptr;   // the null stream: no stream is created
+  HIPCK(hipMemcpyAsync(dR, in->Rtab, z.nTab * 4, hipMemcpyHostToDevice, s), "nz_cutcell_build_noalloc: upload");
+  HIPCK(hipMemcpyAsync(dxs, in->xs, z.npts * 8, hipMemcpyHostToDevice, s), "nz_cutcell_build_noalloc: upload");
+  HIPCK(hipMemcpyAsync(drs, in->rs, z.npts * 8, hipMemcpyHostToDevice, s), "nz_cutcell_build_noalloc: upload");
+  HIPCK(hipMemcpyAsync(dsol, sol, nsolBytes, hipMemcpyHostToDevice, s), "nz_cutcell_build_noalloc: upload");
+  HIPCK(enqueue_build(p, b, s), "nz_cutcell_build_noalloc: kernels");
+  HIPCK(hipMemcpyAsync(mask, b.mask, z.cells, hipMemcpyDeviceToHost, s), "nz_cutcell_build_noalloc: read-back");
+  HIPCK(hipMemcpyAsync(geoA, b.geoA, z.lattice * 16, hipMemcpyDeviceToHost, s), "nz_cutcell_build_noalloc: read-back");
+  HIPCK(hipMemcpyAsync(geoG, b.geoG, z.cells * 16, hipMemcpyDeviceToHost, s), "nz_cutcell_build_noalloc: read-back");
+  HIPCK(hipMemcpyAsync(geoM, b.geoM, z.cells * 16, hipMemcpyDeviceToHost, s), "nz_cutcell_build_noalloc: read-back");
+  if (kappa) HIPCK(hipMemcpyAsync(kappa, b.kap, z.cells * 4, hipMemcpyDeviceToHost, s), "nz_cutcell_build_noalloc: read-back");
+  if (Ax) HIPCK(hipMemcpyAsync(Ax, b.Ax, z.lattice * 4, hipMemcpyDeviceToHost, s), "nz_cutcell_build_noalloc: read-back");
+  if (Ar) HIPCK(hipMemcpyAsync(Ar, b.Ar, z.lattice * 4, hipMemcpyDeviceToHost, s), "nz_cutcell_build_noalloc: read-back");
+  HIPCK(hipMemcpyAsync(cnt, b.counts, 16, hipMemcpyDeviceToHost, s), "nz_cutcell_build_noalloc: read-back");
+  HIPCK(hipStreamSynchronize(s), "nz_cutcell_build_noalloc: sync");
+  if (counts) { counts[0] = (int32_t)cnt[0]; counts[1] = (int32_t)cnt[1]; }
+done:
   return rc;
 }

@@ -211,6 +211,7 @@ SIGNATURES = {
     "nz_setup_get_probes": (C.c_int, [vp, P(NzProbes)]),
     "nz_setup_get_cutcell_input": (C.c_int, [vp, P(NzCutcellIn)]),
     "nz_cutcell_build": (C.c_int, [P(NzCutcellIn), u8p, fp, fp, fp, fp, fp, fp, P(C.c_int32)]),
+    "nz_cutcell_build_noalloc": (C.c_int, [P(NzCutcellIn), u8p, fp, fp, fp, fp, fp, fp, P(C.c_int32)]),
     "nz_setup_radius_at": (C.c_double, [vp, C.c_double]),
     "nz_wall_separation": (C.c_int, [vp, fp, C.c_double, P(NzSeparation)]),
     "nz_summerfield_point": (C.c_int, [vp, C.c_double, C.c_double, C.c_double, dp]),
