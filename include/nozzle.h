@@ -331,6 +331,10 @@ typedef struct nz_config {
   double ambP, ambT;          /* > 0: ambient pressure (Pa) / temperature (K) instead of ISA(alt) */
   int32_t turbulence;         /* 0: eddy-viscosity length scale l = 0 everywhere */
   int32_t deviceGeometry;     /* 1: the cut-cell geometry is built on the GPU (nz_cutcell_build) */
+  /* drop-size law of the injector table (0, 0: the reference's per-element sizes, jitter
+   * uniform in log over +-0.7, spray.js:53): median drop dropD0Frac x the orifice diameter,
+   * lognormal spread dropSigma (written to the table's third row, y) */
+  double dropD0Frac, dropSigma;
 } nz_config;
 
 typedef struct nz_setup nz_setup;

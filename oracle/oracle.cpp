@@ -916,7 +916,16 @@ struct Oracle {
         const v4<T> e1 = load4(tab, t), e2 = load4(tab, 256 + t), e3 = load4(tab, 512 + t);
         const T ang = e1.z + (rnd(id, 2u) - T(0.5)) * T(2) * e2.z;
         const T r = std::max(e1.y + (rnd(id, 3u) - T(0.5)) * e2.w, T(0.05));
-        const T d = e2.y * std::exp((rnd(id, 4u) - T(0.5)) * T(1.4));
+        T d;
+        if (e3.y > T(0)) {
+          // lognormal drop size (injector-table option of this library, not the reference's):
+          // median e2.y, spread e3.y, Box-Muller normal deviate from draws 4 and 6
+          const T u1 = T(1) - rnd(id, 4u), u2 = rnd(id, 6u);
+          const T nrm = std::sqrt(T(-2) * std::log(u1)) * std::cos(T(6.2831853071795865) * u2);
+          d = e2.y * std::exp(e3.y * nrm);
+        } else {
+          d = e2.y * std::exp((rnd(id, 4u) - T(0.5)) * T(1.4));
+        }
         store4(oA, idx, mk(e1.x, r, e1.w * std::cos(ang), e1.w * std::sin(ang)));
         store4(oB, idx, mk(d, Tp<T>(P.sprayMhat) * dt / T(perStep), e2.x, e3.x));
         store4(oC, idx, b.y > T(0) ? mk(b.y, b.y * a.z, b.y * a.w, T(0)) : mk(T(0), T(0), T(0), T(0)));

@@ -113,7 +113,16 @@ __global__ void __launch_bounds__(256) k_spray(const DevInst* __restrict__ insts
       const float* e3 = I.tab + (512 + tt) * 4;
       const R ang = R(e1[2]) + (R(rnd(id, 2u)) - R(0.5)) * R(2) * R(e2[2]);
       const R r = fmax(R(e1[1]) + (R(rnd(id, 3u)) - R(0.5)) * R(e2[3]), R(0.05f));
-      const R d = R(e2[1]) * expR((R(rnd(id, 4u)) - R(0.5)) * R(1.4f));
+      R d;
+      if (e3[1] > 0.f) {
+        // lognormal drop size (an injector-table option): median e2.y, spread e3.y, the normal
+        // deviate by Box-Muller from draws 4 and 6
+        const R u1 = R(1) - R(rnd(id, 4u)), u2 = R(rnd(id, 6u));
+        const R nrm = sq(R(-2) * logR(u1)) * cosR(R(6.2831853071795865) * u2);
+        d = R(e2[1]) * expR(R(e3[1]) * nrm);
+      } else {
+        d = R(e2[1]) * expR((R(rnd(id, 4u)) - R(0.5)) * R(1.4f));   // spray.js:53
+      }
       o[0] = e1[0]; o[1] = r; o[2] = R(e1[3]) * cosR(ang); o[3] = R(e1[3]) * sinR(ang);
       o[4] = d; o[5] = R(c.sprayMhat) * dt / R(perStep); o[6] = e2[0]; o[7] = e3[0];
       if (by > R(0)) { dmo = by; dpx = by * az; dpy = by * aw; }

@@ -181,7 +181,11 @@ int nz_set_geometry(nz_solver* s, const uint8_t* mask, const float* geoA, const 
   if (!mask || !geoA || !geoG || !geoM) return nz_fail(-1, "nz_set_geometry: null array");
   HIPCHK(hipStreamSynchronize(s->stream), "nz_set_geometry: sync");
   if (int rc = nz::inst_set_geometry(s->I, mask, geoA, geoG, geoM, jWall)) return rc;
-  return send_inst(s);
+  if (int rc = send_inst(s)) return rc;
+  // a running flow carries on (app.js:254-270): the next time step comes from the current
+  // state under the new mask, as the reference's reduction takes it every step
+  if (s->haveState) return nz::inst_cfl_init(s->I, s->stream);
+  return 0;
 }
 
 int nz_set_mask(nz_solver* s, const uint8_t* mask) {

@@ -739,6 +739,10 @@ static std::vector<float> buildInjectorTable(const nz_config& cfg, const Contour
       else add(1, mo * o[b++], x0, rs[k], 0, Vo, 0.03 * Do, 4 * deg, 0.6 * Do, 0);
     }
   }
+  // drop-size law (an extension; 0 keeps the reference's): every element's median drop at
+  // dropD0Frac x the orifice, lognormal spread dropSigma (the film stays as the reference has it)
+  if (cfg.dropD0Frac > 0)
+    for (El& e : el) e.drop = cfg.dropD0Frac * Do;
   if (film > 0) add(0, mf * film, x0, 0.97 * c.Rc, 2 * deg, 0.5 * Vf, 80e-6, 1 * deg, 0.015 * c.Rc, 0);
   // largest-remainder fill of 256 slots; the reference sorts with
   // Array.prototype.sort, which is stable, so ties keep element order
@@ -768,6 +772,7 @@ static std::vector<float> buildInjectorTable(const nz_config& cfg, const Contour
       r1[0] = f32(e.type); r1[1] = f32(e.drop); r1[2] = f32(e.cone); r1[3] = f32(e.width);
       float* r2 = &tab[(size_t)(2 * N + t) * 4];
       r2[0] = f32(e.xi);
+      r2[1] = cfg.dropSigma > 0 ? f32(cfg.dropSigma) : 0.f;   // > 0: lognormal drop-size draw
     }
   return tab;
 }

@@ -58,7 +58,7 @@ class NzConfig(C.Structure):
         ("film", C.c_double), ("baffles", C.c_double), ("baffleH", C.c_double), ("cavD", C.c_double), ("cavW", C.c_double),
         ("dx", C.c_double), ("hllBlend", C.c_double),
         ("convAngle", C.c_double), ("ambP", C.c_double), ("ambT", C.c_double), ("turbulence", C.c_int32),
-        ("deviceGeometry", C.c_int32),
+        ("deviceGeometry", C.c_int32), ("dropD0Frac", C.c_double), ("dropSigma", C.c_double),
     ]
 
 

@@ -60,6 +60,12 @@ def hot_bell(Nx=768, **over):
     return apply(cfg, over)
 
 
+# the kerolox workload's own drop-size law: median drop 0.1 x the 1.5 mm orifice, lognormal
+# spread 0.3 (the reference's law, 0.12 x the orifice with a +-0.7 jitter uniform in log, is the
+# default for parity); kerolox_injector(**CONFIG_DROPS) selects it
+CONFIG_DROPS = {"dropD0Frac": 0.1, "dropSigma": 0.3}
+
+
 def kerolox_injector(Nx=768, **over):
     cfg = default_config(prop="kerolox", Tc=3670, gamma=1.22, Mw=22.0, qAb=6.8, sAir=1.8, Pc=70e5, Dt=0.030,
                          De=0.085, type="bell", bellPct=80, CR=5, chamberLen=6, domainL=0.5, Nx=Nx, aspect=256 / 768,
@@ -96,12 +102,13 @@ def batch_instance(rng, Nx=None):
     return cfg
 
 
-def injector_instance(rng):
+def injector_instance(rng, kind=None):
     """One spray request of the serving mix: 384 x 128 kerolox injector, 16,384
     parcels, element type uniform over showerhead / coaxial / impinging / pintle,
     mixture ratio uniform in [2, 3.5]; chamber and nozzle of the reference's
     injector presets (injImpinge, physics.js:246-251)."""
-    kind = ["shower", "coax", "impinge", "pintle"][int(rng.integers(0, 4))]
+    draw = ["shower", "coax", "impinge", "pintle"][int(rng.integers(0, 4))]
+    kind = kind or draw
     cfg = default_config(prop="kerolox", Tc=3670, gamma=1.22, Mw=22.0, qAb=6.8, sAir=1.8, Pc=70e5, Dt=0.030,
                          De=0.085, type="bell", bellPct=80, CR=5, chamberLen=6, Nx=384, aspect=128 / 384,
                          injMode=1, injType=kind, injRings=4 if kind != "pintle" else 1, injDP=20,

@@ -496,12 +496,15 @@ def test_spray_deterministic_rerun(prec):
         assert np.array_equal(x, y)
 
 
-def test_spawn_draws_on_device():
+@pytest.mark.parametrize("law", ["reference", "lognormal"])
+def test_spawn_draws_on_device(law):
     """The parcels the device spawns in its first two steps against the draws of the
     literal pcg_hash table (tests/test_oracle_pins.py): element slot, angle, radius and
-    diameter of each."""
-    from test_oracle_pins import M32, spray_rnd
-    s = SimSetup(configs.kerolox_injector(Nx=160))
+    diameter of each; with the reference's drop-size law and with the lognormal option
+    (sigma 0.3 at 0.1 x the orifice)."""
+    from test_oracle_pins import M32, lognormal_diameter, spray_rnd
+    over = {"dropD0Frac": 0.1, "dropSigma": 0.3} if law == "lognormal" else {}
+    s = SimSetup(configs.kerolox_injector(Nx=160, **over))
     g = GPUSolver()
     g.resize(s.Nx, s.Nr)
     g.spray.setCount(4096)
@@ -518,7 +521,10 @@ def test_spawn_draws_on_device():
             t = int(np.float32(np.float32(spray_rnd(pid, 1)) * np.float32(255.999)))
             ang = tab[0, t, 2] + (spray_rnd(pid, 2) - 0.5) * 2 * tab[1, t, 2]
             r = max(tab[0, t, 1] + (spray_rnd(pid, 3) - 0.5) * tab[1, t, 3], 0.05)
-            d = tab[1, t, 1] * np.exp((spray_rnd(pid, 4) - 0.5) * 1.4)
+            if tab[2, t, 1] > 0:
+                d = lognormal_diameter(tab[1, t, 1], tab[2, t, 1], pid)
+            else:
+                d = tab[1, t, 1] * np.exp((spray_rnd(pid, 4) - 0.5) * 1.4)
             a, b = PA[idx], PB[idx]
             assert a[0] == tab[0, t, 0] and b[2] == tab[1, t, 0] and b[3] == tab[2, t, 0], (idx, t)
             assert a[1] == pytest.approx(r, rel=1e-6)
@@ -562,3 +568,56 @@ def test_derived_display_fields(prec):
             e = rel_l2(out[~wall, ch], ref[~wall, ch].astype(np.float64))
             assert e <= (1e-7 if prec == 64 and f in (9, 13) else tol), (f, ch, e)
     say(f"[parity] derived fields FP{prec}: all {len(DERIVED_FIELDS)} fields within {tol}")
+
+
+@pytest.mark.parametrize("prec,tol", [(32, None), (64, 1e-9)])
+def test_live_remesh(prec, tol):
+    """remesh() (app.js:254-270): a new geometry at the same cell size (a steeper 18-degree
+    cone) handed to setGeometry on a running flow, which carries on. The GPU and the oracle
+    take the same sequence: 150 steps, setGeometry, 60 steps; FP32 at the fixed bar, FP64 to
+    1e-9."""
+    from rnozzle import configs as cf
+    s = SimSetup(cf.cold_cone(Nx=128))
+    g, o = run_pair(s, 150, prec=prec)
+    s2 = SimSetup(cf.apply(cf.cold_cone(Nx=128), {"coneAngle": 18.0, "dx": s.dx}))
+    assert (s2.Nx, s2.Nr) == (s.Nx, s.Nr)
+    assert not np.array_equal(s2.mask, s.mask)        # the wall moved
+    g.setGeometry({"mask": s2.mask, "geoA": s2.geoA, "geoG": s2.geoG, "geoM": s2.geoM, "jWall": s2.jWall})
+    o.set_geometry(s2.mask, s2.geoA, s2.geoG, s2.geoM, s2.jWall)
+    g.step(60, s2.P)
+    o.step(60, s2.P)
+    g.sync()
+    compare_state(g, o, tol if tol else 1e-4, f"cold cone re-meshed to 18 deg, FP{prec}")
+
+
+@pytest.mark.parametrize("viscous", [0, 1])
+def test_nan_heals_to_ambient(viscous):
+    """A NaN planted in a running flow (setState) is healed in one step by the update's NaN
+    guard (solver.js:348): the field is finite again and the GPU agrees with the oracle. On
+    the inviscid flow every cell the guard resets holds exactly the ambient state (with the
+    viscous split step on, the split step then acts on the reset cells as on any other)."""
+    s = SimSetup(configs.apply(configs.cold_cone(Nx=128), {"viscous": viscous}))
+    g, _ = run_pair(s, 100)
+    st = g.readAll()
+    A, B = st["A"].reshape(s.Nr, s.Nx, 4).copy(), st["B"].reshape(s.Nr, s.Nx, 4).copy()
+    fluid = s.mask.reshape(s.Nr, s.Nx) == 0
+    jn, i_n = 5, 40
+    assert fluid[jn, i_n]
+    A[jn, i_n, :] = np.nan
+    g.setState(A.ravel(), B.ravel())
+    o = orc.Oracle(s.Nx, s.Nr, 32)
+    o.set_geometry(s.mask, s.geoA, s.geoG, s.geoM, s.jWall)
+    o.set_state(A.ravel(), B.ravel())
+    g.step(1, s.P)
+    o.step(1, s.P)
+    G = g.readAll()
+    assert np.isfinite(G["A"]).all() and np.isfinite(G["B"]).all()
+    Ao, Bo = o.state()
+    assert np.isfinite(Ao).all() and np.isfinite(Bo).all()
+    amb = np.array(list(s.P.ambCons), np.float32)
+    healed = np.all(G["A"].reshape(s.Nr, s.Nx, 4) == amb, axis=-1) & fluid
+    if not viscous:
+        assert healed[jn, i_n], "the NaN cell did not take the ambient state"
+        assert np.array_equal(healed, np.all(Ao.reshape(s.Nr, s.Nx, 4) == amb, axis=-1) & fluid)
+    gate(prims(G["A"], G["B"]), prims(Ao, Bo), f"NaN planted at ({i_n}, {jn}), viscous={viscous}: "
+         f"{int(healed.sum())} cells at the ambient state")

@@ -614,3 +614,61 @@ def test_respawn_draws():
     want = np.array(counts) / 256.0
     sigma = np.sqrt(want * (1 - want) / n)
     assert np.all(np.abs(freq - want) < 4 * sigma), (freq, want)
+
+
+def lognormal_diameter(d_med, sigma, pid):
+    """The lognormal option's draw: Box-Muller on rnd 4 and 6 of the literal hash."""
+    u1, u2 = 1.0 - spray_rnd(pid, 4), spray_rnd(pid, 6)
+    return d_med * np.exp(sigma * np.sqrt(-2.0 * np.log(u1)) * np.cos(2 * np.pi * u2))
+
+
+def test_lognormal_drop_size_option():
+    """The drop-size option of the injector table (third row y = sigma > 0): each spawned
+    diameter is d_med exp(sigma N), N the Box-Muller normal deviate of the literal hash's
+    draws 4 and 6; over the sample, ln(d / d_med) has mean 0 and standard deviation sigma.
+    The setup writes the option: median at dropD0Frac x the orifice, sigma in the table."""
+    from rnozzle import configs
+    from rnozzle.sim_setup import SimSetup
+    s = SimSetup(configs.kerolox_injector(Nx=160, dropD0Frac=0.1, dropSigma=0.3))
+    tab = s.injector.reshape(3, 256, 4)
+    Do = 1.5e-3
+    film = tab[1, :, 1] == np.float32(80e-6)
+    assert np.all(tab[1, ~film, 1] == np.float32(0.1 * Do)) and np.all(tab[2, :, 1] == np.float32(0.3))
+    ref = SimSetup(configs.kerolox_injector(Nx=160)).injector.reshape(3, 256, 4)
+    assert np.all(ref[2, :, 1] == 0)                      # the reference's tables leave it 0
+    NP, steps, sigma = 1024, 500, 0.3
+    Nx, Nr = 16, 8
+    solid = np.zeros((Nr, Nx), bool)
+    solid[:, 0] = solid[:, Nx - 1] = True
+    solid[Nr - 1, :] = True
+    mask, geoA, geoG, geoM = cells_geometry(solid)
+    A, B = state_from_prims(np.full((Nr, Nx), 1.0), np.zeros((Nr, Nx)), np.zeros((Nr, Nx)), np.full((Nr, Nx), 1.0))
+    P = uniform_params()
+    P.inj = 1
+    P.sprayMhat = 1e-4
+    P.sprayTScale = 1e-6
+    P.fuelVec[:] = (800.0, 2000.0, 3e5, 0.02)
+    P.oxVec[:] = (1100.0, 2000.0, 2e5, 0.015)
+    t = np.zeros((3, 256, 4), np.float32)
+    t[0, :, 0], t[0, :, 1], t[0, :, 3] = 3.0, 2.0, 0.4
+    t[1, :, 1] = 2e-5
+    t[2, :, 1] = sigma
+    o = orc.Oracle(Nx, Nr, 64)
+    o.set_geometry(mask, geoA, geoG, geoM)
+    o.set_parcels(NP)
+    o.set_injector(t.ravel())
+    o.set_state(A, B)
+    logs = []
+    for s_ in range(steps):
+        o.step(1, P)
+        _, PB = (x.reshape(-1, 4) for x in o.parcels_f64(NP))
+        for k in range(4):
+            idx = (4 * s_ + k) % NP
+            pid = (idx ^ (((1 + s_) * 2654435761) & M32)) & M32
+            want = lognormal_diameter(float(np.float32(2e-5)), float(np.float32(sigma)), pid)
+            assert PB[idx, 0] == pytest.approx(want, rel=1e-6)
+            logs.append(np.log(PB[idx, 0] / float(np.float32(2e-5))))
+    logs = np.array(logs)
+    n = len(logs)
+    assert abs(np.mean(logs)) < 4 * sigma / np.sqrt(n)
+    assert abs(np.std(logs) - sigma) < 4 * sigma / np.sqrt(2 * n)
