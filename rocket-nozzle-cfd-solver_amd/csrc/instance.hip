@@ -316,13 +316,15 @@ int inst_set_shape(Inst& I, int shape) {
   int TX = 0, TY = 0;
   if (!nzk::tile_shape(I.prec == 64, shape, &TX, &TY)) return nz_fail(-1, "unknown tile shape");
   if (shape == I.shape) return 0;
+  // the last step's partials, of the old tiling, folded into the form every tiling reads
+  HIPCHK(nzk::launch_part_canon(I.part, I.tilesX * I.tilesY, (int)I.partCap, nullptr), "partials");
+  HIPCHK(hipDeviceSynchronize(), "partials");
   I.shape = shape;
   I.TX = TX;
   I.TY = TY;
   I.tilesX = (I.Nx + TX - 1) / TX;
   I.tilesY = (I.Nr + TY - 1) / TY;
   const size_t tiles = (size_t)I.tilesX * I.tilesY;
-  // the partials (capacity for every shape) keep the last step's values
   hfree(I.exStart);
   hfree(I.fcStart);
   HIPCHK(hipMalloc(&I.exStart, (tiles + 1) * sizeof(int)), "hipMalloc extra starts");
@@ -377,7 +379,7 @@ DevInst inst_desc(const Inst& I) {
 }
 
 int inst_cfl_init(Inst& I, hipStream_t s) {
-  HIPCHK(nzk::launch_cfl_init(I.prec == 64, inst_desc(I), s), "cfl init");
+  HIPCHK(nzk::launch_cfl_init(I.prec == 64, inst_desc(I), (int)I.partCap, s), "cfl init");
   return 0;
 }
 

@@ -41,7 +41,7 @@ __global__ void __launch_bounds__(256) k_close(const DevInst* __restrict__ insts
     if (threadIdx.x == 0) s->flagsSeen = s->flags;
     return;
   }
-  fold_groups<256>(J.part, s->nPart, threadIdx.x, W, R);
+  fold_groups<256>(J.part, (uint32_t)(J.tilesX * J.tilesY), threadIdx.x, W, R);
   __syncthreads();
   double w, r;
   fold_combine(W, R, w, r);
@@ -253,16 +253,31 @@ __global__ void __launch_bounds__(256) k_cfl_partial(DevInst I) {
   const double wb = block_max<256>(w, red);
   if (threadIdx.x == 0) I.part[blockIdx.x] = make_double2(wb, 0.0);
 }
-__global__ void __launch_bounds__(256) k_cfl_reduce(DevInst I, int nblk) {
+__global__ void __launch_bounds__(256) k_cfl_reduce(DevInst I, int nblk, int cap) {
   __shared__ double red[16];
   double w = 0.0;
   for (int k = threadIdx.x; k < nblk; k += 256) w = fmax(w, I.part[k].x);
   const double wb = block_max<256>(w, red);
+  __syncthreads();
+  // the partials folded into entry 0, zeros after it (valid for every tiling)
+  for (int k = 1 + threadIdx.x; k < cap; k += 256) I.part[k] = make_double2(0.0, 0.0);
   if (threadIdx.x == 0) {
+    I.part[0] = make_double2(wb, 0.0);
     I.st->wsnap = (unsigned long long)__double_as_longlong(wb);
     I.st->wacc = 0ull;
-    I.st->nPart = (uint32_t)nblk;  // the next stage 1 folds these
   }
+}
+
+// the n partials of a tiling folded (in the stage kernels' fixed order) into entry 0, with
+// zeros up to cap: the form every tiling reads alike (before a tile-shape change)
+__global__ void __launch_bounds__(256) k_part_canon(double2* __restrict__ part, int n, int cap) {
+  __shared__ double W[4], R[4];
+  fold_groups<256>(part, (uint32_t)n, threadIdx.x, W, R);
+  __syncthreads();
+  double w, r;
+  fold_combine(W, R, w, r);
+  for (int k = 1 + threadIdx.x; k < cap; k += 256) part[k] = make_double2(0.0, 0.0);
+  if (threadIdx.x == 0) part[0] = make_double2(w, r);
 }
 
 // FS_RESCALE (solver.js:425-436), in place
@@ -579,14 +594,18 @@ hipError_t launch_spray(bool fp64, const DevInst* d, int ninst, int nblk, hipStr
   }
   return hipGetLastError();
 }
-hipError_t launch_cfl_init(bool fp64, const DevInst& I, hipStream_t s) {
+hipError_t launch_part_canon(double2* part, int n, int cap, hipStream_t s) {
+  hipLaunchKernelGGL(k_part_canon, dim3(1), dim3(256), 0, s, part, n, cap);
+  return hipGetLastError();
+}
+hipError_t launch_cfl_init(bool fp64, const DevInst& I, int partCap, hipStream_t s) {
   // one partial per block: at most ceil(N / 256), which no shape's tile count undercuts
   // (the smallest tile holds 128 cells in FP32, 64 in FP64), so they fit the partials array
   const long long N = (long long)I.Nx * I.Nr;
   const int nblk = (int)std::min<long long>((N + 255) / 256, 1024);
   if (fp64) hipLaunchKernelGGL((k_cfl_partial<double>), dim3(nblk), dim3(256), 0, s, I);
   else hipLaunchKernelGGL((k_cfl_partial<float>), dim3(nblk), dim3(256), 0, s, I);
-  hipLaunchKernelGGL(k_cfl_reduce, dim3(1), dim3(256), 0, s, I, nblk);
+  hipLaunchKernelGGL(k_cfl_reduce, dim3(1), dim3(256), 0, s, I, nblk, partCap);
   return hipGetLastError();
 }
 hipError_t launch_rescale(bool fp64, const DevInst& I, float fp, float fT, float du, hipStream_t s) {

@@ -80,7 +80,7 @@ struct InstState {
   double resLast;  // RMS of the last completed step
   double nGroups;  // number of fluid groups (residual normalisation)
   uint32_t flags;  // bit 0: instance inactive (skipped); stopped on bit 1: the residual, 2: time, 3: step count
-  uint32_t nPart;  // tile partials (wave speed, residual) the last stage-2 launch wrote (or the CFL set-up)
+  uint32_t pad0_;
   uint32_t flagsSeen;  // `flags` as the last stage 2 (or close kernel) read them: stage 1, whose first
                        // workgroup writes `flags`, reads this copy instead
   uint32_t pad_;
@@ -88,8 +88,11 @@ struct InstState {
 };
 
 // The next step's stage 1 folds the tile partials of the last stage 2 into the time step
-// itself (one wave per workgroup reads them) when there are at most this many; with more,
-// a close kernel folds them after every step.
+// itself (every workgroup reads them) when there are at most this many; with more, a close
+// kernel folds them after every step. The partials array always holds the current tiling's
+// count of valid entries (tilesX * tilesY): stage 2 writes one per tile, and the CFL set-up
+// and a tile-shape change leave it folded into entry 0 with zeros after it, which every
+// tiling reads alike (a zero changes neither the max nor the sum).
 constexpr uint32_t FUSED_CLOSE_MAX = 2048;
 
 struct DevInst {

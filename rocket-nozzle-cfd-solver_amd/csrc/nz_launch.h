@@ -17,7 +17,10 @@ hipError_t launch_stage(int stage, bool phys, bool fp64, int shape, const DevIns
 hipError_t launch_close(const DevInst* d, int ninst, hipStream_t s);
 hipError_t launch_spray(bool fp64, const DevInst* d, int ninst, int nblk, hipStream_t s, hipEvent_t e0 = nullptr,
                         hipEvent_t e1 = nullptr);
-hipError_t launch_cfl_init(bool fp64, const DevInst& I, hipStream_t s);
+// wave-speed max of the current state; leaves the partials folded into entry 0, zeros up to partCap
+hipError_t launch_cfl_init(bool fp64, const DevInst& I, int partCap, hipStream_t s);
+// the n partials of a tiling folded into entry 0, zeros up to cap (before a tile-shape change)
+hipError_t launch_part_canon(double2* part, int n, int cap, hipStream_t s);
 hipError_t launch_rescale(bool fp64, const DevInst& I, float fp, float fT, float du, hipStream_t s);
 hipError_t launch_wall_row(bool fp64, const DevInst& I, const float* jw, float* out, hipStream_t s);
 hipError_t launch_t2(bool fp64, const DevInst& I, float* out, hipStream_t s);

@@ -65,14 +65,39 @@ __device__ __forceinline__ void step_info(const Consts& c, const InstState& s, d
 // divides 256) runs virtual lanes tid, tid + NT, ...; the loads of four consecutive
 // partials of a lane are issued together. fold_groups leaves the group results in W[4],
 // R[4] (LDS), which fold_combine reads after a barrier.
+// The fold in two parts, so that its first loads can be issued with a tile's own loads and
+// consumed later: fold_issue loads each virtual lane's first four partials, fold_finish folds
+// them, loads and folds the rest (more than 1024 partials), and leaves the group results.
 template <int NT>
-__device__ __forceinline__ void fold_groups(const double2* __restrict__ part, uint32_t n, int tid, double* W, double* R) {
+struct FoldLoads {
+  double2 p[256 / NT][4];
+};
+template <int NT>
+__device__ __forceinline__ void fold_issue(const double2* __restrict__ part, uint32_t n, int tid, FoldLoads<NT>& f) {
   static_assert(256 % NT == 0 && NT % 64 == 0, "fold: the workgroup size must divide 256");
 #pragma unroll
   for (int m = 0; m < 256 / NT; ++m) {
     const uint32_t v = (uint32_t)(tid + m * NT);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const uint32_t k = v + 256u * j;
+      f.p[m][j] = k < n ? part[k] : make_double2(0.0, 0.0);
+    }
+  }
+}
+template <int NT>
+__device__ __forceinline__ void fold_finish(const double2* __restrict__ part, uint32_t n, int tid, const FoldLoads<NT>& f,
+                                            double* W, double* R) {
+#pragma unroll
+  for (int m = 0; m < 256 / NT; ++m) {
+    const uint32_t v = (uint32_t)(tid + m * NT);
     double w = 0.0, r = 0.0;
-    for (uint32_t k0 = v; k0 < n; k0 += 1024) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      w = fmax(w, f.p[m][j].x);
+      r += f.p[m][j].y;
+    }
+    for (uint32_t k0 = v + 1024; k0 < n; k0 += 1024) {
       double2 p[4];
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
@@ -91,6 +116,12 @@ __device__ __forceinline__ void fold_groups(const double2* __restrict__ part, ui
     }
     if ((v & 63u) == 0u) { W[v >> 6] = w; R[v >> 6] = r; }
   }
+}
+template <int NT>
+__device__ __forceinline__ void fold_groups(const double2* __restrict__ part, uint32_t n, int tid, double* W, double* R) {
+  FoldLoads<NT> f;
+  fold_issue<NT>(part, n, tid, f);
+  fold_finish<NT>(part, n, tid, f, W, R);
 }
 __device__ __forceinline__ void fold_combine(const double* W, const double* R, double& w, double& r) {
   w = fmax(fmax(W[0], W[1]), fmax(W[2], W[3]));
@@ -354,16 +385,16 @@ __global__ void __launch_bounds__(TX* TY, (STAGE == 2 && PHYS) ? 1 : W) k_stage(
   // launch in which no workgroup reads it: stage 1's first workgroup sets the stop bits in
   // `flags`, which stage 2 reads, while stage 1 reads `flagsSeen`, the copy that stage 2's
   // first workgroup (or the close kernel) writes.
-  bool was = false, active = false, fused = false;
+  bool was = false, active = false;
   R cur[4] = {R(0), R(0), R(0), R(0)};
   R dt = R(0), tsel = R(0);
-  uint32_t nPart = 0u;
+  // the last step's partials: one per tile of this tiling (see FUSED_CLOSE_MAX)
+  const uint32_t nPart = (uint32_t)(I.tilesX * I.tilesY);
+  const bool fused = nPart <= FUSED_CLOSE_MAX;
   auto step_state = [&]() {
     was = !((STAGE == 1 ? st->flagsSeen : st->flags) & 1u);
     active = was;
     if (STAGE == 1) {
-      nPart = st->nPart;
-      fused = nPart <= FUSED_CLOSE_MAX;
       tsel = R(st->last[2]);
       if (was) {
         const uint32_t f = stop_bits(c, *st, INFINITY);
@@ -398,7 +429,6 @@ __global__ void __launch_bounds__(TX* TY, (STAGE == 2 && PHYS) ? 1 : W) k_stage(
           st->head = (st->head + 4u) % (uint32_t)c.NP;
           st->seed = st->seed + 1u;
         }
-        st->nPart = (uint32_t)(I.tilesX * I.tilesY);
       }
     }
   };
@@ -462,13 +492,17 @@ __global__ void __launch_bounds__(TX* TY, (STAGE == 2 && PHYS) ? 1 : W) k_stage(
       if (dirR ? (fi < Nx && fj <= Nr) : (fi <= Nx && fj < Nr)) area[r] = face_area(I, dirR, fi, fj);
     }
   }
+  // the last step's partials (stage 1): loaded with the tile, folded once the tile's
+  // primitives are in LDS, so that their round trip overlaps the tile's
+  // (tiles of fewer than 256 threads hold more partials per thread: they load them late)
+  constexpr bool EARLY_FOLD = NT >= 256;
+  FoldLoads<NT> fl;
+  if (STAGE == 1 && EARLY_FOLD && fused) fold_issue<NT>(I.part, nPart, tid, fl);
   if constexpr (W < 4) step_state();
 
   double wloc = 0.0;
   double rloc = 0.0;
   if (active) {
-    // the last step's partials are folded while the tile's loads are in flight
-    if (STAGE == 1 && fused) fold_groups<NT>(I.part, nPart, tid, sm.foldW, sm.foldR);
 #pragma unroll
     for (int r = 0; r < CPT; ++r) {
       const int idx = tid + r * NT;
@@ -497,6 +531,10 @@ __global__ void __launch_bounds__(TX* TY, (STAGE == 2 && PHYS) ? 1 : W) k_stage(
       const Prim<R> q = prim_at<R>(I, Uin, fcell.x, fcell.y, tsel);
       sm.PF[0][e] = q.r; sm.PF[1][e] = q.u; sm.PF[2][e] = q.v; sm.PF[3][e] = q.p;
       sm.PF[4][e] = q.Z; sm.PF[5][e] = q.F; sm.PF[6][e] = q.G; sm.PF[7][e] = q.O;
+    }
+    if (STAGE == 1 && fused) {
+      if constexpr (EARLY_FOLD) fold_finish<NT>(I.part, nPart, tid, fl, sm.foldW, sm.foldR);
+      else fold_groups<NT>(I.part, nPart, tid, sm.foldW, sm.foldR);
     }
     if (nF > 0) {  // uniform over the workgroup
       for (int w = tid; w < S::WW * S::WH; w += NT) sm.fmap[w / S::WW][w % S::WW] = 0xFF;

@@ -321,6 +321,8 @@ struct Snapshot {
 };
 int snap_take(nz_solver* s, Snapshot& z) {
   const nz::Inst& I = s->I;
+  // the partials in the form every tiling reads, so that a restore under another shape holds
+  HIPCHK(nzk::launch_part_canon(I.part, I.tilesX * I.tilesY, (int)I.partCap, s->stream), "snapshot partials");
   const size_t us = (size_t)I.Ns * 8 * I.rsz, ps = (size_t)I.NP * 8 * 4, ss = (size_t)I.Ns * 32;
   HIPCHK(hipMalloc(&z.U0, us), "snapshot");
   HIPCHK(hipMalloc(&z.U1, us), "snapshot");
@@ -399,7 +401,7 @@ int nz_rescale_ambient(nz_solver* s, float fp, float fT, float du, const nz_para
   if (!P) return nz_fail(-1, "nz_rescale_ambient: null params");
   if (int rc = apply_params(s, P)) return rc;
   HIPCHK(nzk::launch_rescale(s->prec == 64, s->h, fp, fT, du, s->stream), "rescale");
-  HIPCHK(nzk::launch_cfl_init(s->prec == 64, s->h, s->stream), "cfl init");
+  HIPCHK(nzk::launch_cfl_init(s->prec == 64, s->h, (int)s->I.partCap, s->stream), "cfl init");
   return 0;
 }
 
