@@ -75,6 +75,9 @@ struct nz_batch {
   InstState* stHost = nullptr;   // pinned copy read after each run
   DevInst* dtab = nullptr;       // device descriptor table (resident order)
   DevInst* htab = nullptr;       // pinned host copy for the upload
+  unsigned short* dti = nullptr; // device: the resident index of every launch tile
+  int dtiCap = 0;
+  std::vector<unsigned short> hti;
   std::vector<int> freeSlots;
   std::vector<std::unique_ptr<Resident>> res;
   std::deque<std::unique_ptr<Job>> queue;
@@ -158,8 +161,25 @@ int upload_table(nz_batch* b) {
   }
   b->ntiles = t;
   b->npblk = pb;
-  if (!b->res.empty())
+  b->hti.resize((size_t)t);
+  for (size_t n = 0; n < b->res.size(); ++n) {
+    const DevInst& d = b->htab[n];
+    std::fill(b->hti.begin() + d.tile0, b->hti.begin() + d.tile0 + d.tilesX * d.tilesY, (unsigned short)n);
+  }
+  if (t > b->dtiCap) {
+    HIPCHK(hipStreamSynchronize(b->stream), "batch tile table sync");
+    const int cap = std::max(t, 2 * b->dtiCap);
+    if (b->dti) (void)hipFree(b->dti);
+    b->dti = nullptr;
+    b->dtiCap = 0;
+    HIPCHK(hipMalloc(&b->dti, (size_t)cap * sizeof(unsigned short)), "batch tile table");
+    b->dtiCap = cap;
+  }
+  if (!b->res.empty()) {
     HIPCHK(hipMemcpyAsync(b->dtab, b->htab, b->res.size() * sizeof(DevInst), hipMemcpyHostToDevice, b->stream), "batch table");
+    HIPCHK(hipMemcpyAsync(b->dti, b->hti.data(), (size_t)t * sizeof(unsigned short), hipMemcpyHostToDevice, b->stream),
+           "batch tile table");
+  }
   HIPCHK(hipStreamSynchronize(b->stream), "batch table sync");
   b->tableDirty = false;
   drop_graph(b);
@@ -174,9 +194,10 @@ int enqueue_steps(nz_batch* b, int steps) {
   bool closeEach = false;
   for (const auto& r : b->res) closeEach |= (unsigned)(r->I.tilesX * r->I.tilesY) > nzk::FUSED_CLOSE_MAX;
   for (int k = 0; k < steps; ++k) {
-    HIPCHK(nzk::launch_stage(1, false, fp64, b->shape, b->dtab, n, b->ntiles, b->stream), "batch stage 1");
+    const int tx0 = n ? b->res[0]->I.tilesX : 1;
+    HIPCHK(nzk::launch_stage(1, false, fp64, b->shape, b->dtab, n, b->ntiles, tx0, b->dti, b->stream), "batch stage 1");
     if (b->npblk) HIPCHK(nzk::launch_spray(fp64, b->dtab, n, b->npblk, b->stream), "batch spray");
-    HIPCHK(nzk::launch_stage(2, b->phys, fp64, b->shape, b->dtab, n, b->ntiles, b->stream), "batch stage 2");
+    HIPCHK(nzk::launch_stage(2, b->phys, fp64, b->shape, b->dtab, n, b->ntiles, tx0, b->dti, b->stream), "batch stage 2");
     if (closeEach || k == steps - 1) HIPCHK(nzk::launch_close(b->dtab, n, b->stream), "batch close");
   }
   return 0;
@@ -299,6 +320,7 @@ int nz_batch_destroy(nz_batch* b) {
   b->res.clear();
   (void)hipFree(b->st);
   (void)hipFree(b->dtab);
+  if (b->dti) (void)hipFree(b->dti);
   (void)hipHostFree(b->stHost);
   (void)hipHostFree(b->htab);
   (void)hipStreamDestroy(b->stream);

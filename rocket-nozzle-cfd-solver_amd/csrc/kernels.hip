@@ -543,9 +543,9 @@ hipError_t launch_set_inst(DevInst* dst, const DevInst& src, hipStream_t s) {
   return hipGetLastError();
 }
 
-#define NZ_DECL_STAGE(name, R, TX, TY, W)                                                                  \
-  hipError_t stage_launch_##name(int stage, bool phys, const DevInst* d, int ninst, int ntiles, hipStream_t s, \
-                                 hipEvent_t e0, hipEvent_t e1);
+#define NZ_DECL_STAGE(name, R, TX, TY, W)                                                                       \
+  hipError_t stage_launch_##name(int stage, bool phys, const DevInst* d, int ninst, int ntiles, int chunk,        \
+                                 const unsigned short* tileInst, hipStream_t s, hipEvent_t e0, hipEvent_t e1);
 NZ_STAGE_SHAPES_F32(NZ_DECL_STAGE)
 NZ_STAGE_SHAPES_F64(NZ_DECL_STAGE)
 #undef NZ_DECL_STAGE
@@ -553,7 +553,7 @@ NZ_STAGE_SHAPES_F64(NZ_DECL_STAGE)
 namespace {
 struct ShapeEntry {
   int tx, ty, w;
-  hipError_t (*launch)(int, bool, const DevInst*, int, int, hipStream_t, hipEvent_t, hipEvent_t);
+  hipError_t (*launch)(int, bool, const DevInst*, int, int, int, const unsigned short*, hipStream_t, hipEvent_t, hipEvent_t);
 };
 #define NZ_SHAPE_ENTRY(name, R, TX, TY, W) {TX, TY, W, stage_launch_##name},
 const ShapeEntry kShapes32[] = {NZ_STAGE_SHAPES_F32(NZ_SHAPE_ENTRY)};
@@ -575,10 +575,12 @@ bool tile_shape(bool fp64, int shape, int* tx, int* ty, int* waves) {
 }
 
 hipError_t launch_stage(int stage, bool phys, bool fp64, int shape, const DevInst* d, int ninst, int ntiles,
-                        hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
+                        int tilesX0, const unsigned short* tileInst, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
   if (shape < 0 || shape >= tile_shape_count(fp64)) return hipErrorInvalidValue;
+  if (ninst > 1 && !tileInst) return hipErrorInvalidValue;
   const ShapeEntry& e = fp64 ? kShapes64[shape] : kShapes32[shape];
-  return e.launch(stage, phys, d, ninst, ntiles, s, e0, e1);
+  const int chunk = tilesX0 * NZ_XCD_CHUNK_ROWS > 1 ? tilesX0 * NZ_XCD_CHUNK_ROWS : 1;
+  return e.launch(stage, phys, d, ninst, ntiles, chunk, tileInst, s, e0, e1);
 }
 hipError_t launch_close(const DevInst* d, int ninst, hipStream_t s) {
   hipLaunchKernelGGL(k_close, dim3(ninst), dim3(256), 0, s, d, ninst);

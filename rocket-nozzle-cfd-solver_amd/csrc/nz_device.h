@@ -95,7 +95,8 @@ struct InstState {
 // tiling reads alike (a zero changes neither the max nor the sum).
 constexpr uint32_t FUSED_CLOSE_MAX = 2048;
 
-struct DevInst {
+// 64-byte aligned, so a descriptor spans exactly sizeof(DevInst) / 64 scalar-cache lines
+struct alignas(64) DevInst {
   int Nx, Nr;
   int tilesX, tilesY, tile0;  // tile grid of this instance inside a launch
   int pblk0, npblk;           // parcel blocks of this instance inside a spray launch
@@ -122,6 +123,19 @@ struct DevInst {
   Consts c;
 };
 
+// A workgroup's first touch of its instance descriptor: one scalar load per 64-byte line,
+// all issued together and waited for once, so that the field loads that follow (which
+// the compiler places next to their uses, each with its own wait) hit the scalar cache
+// instead of each paying a cold round trip to memory at kernel start.
+__device__ __forceinline__ void touch_desc(const DevInst* __restrict__ d) {
+  const unsigned* w = reinterpret_cast<const unsigned*>(d);
+  unsigned v[sizeof(DevInst) / 64];
+#pragma unroll
+  for (int l = 0; l < (int)(sizeof(DevInst) / 64); ++l) v[l] = __builtin_nontemporal_load(w + 16 * l);
+#pragma unroll
+  for (int l = 0; l < (int)(sizeof(DevInst) / 64); ++l) asm volatile("" ::"s"(v[l]));
+}
+
 // Global-memory access: a uniform 64-bit base (SGPRs) plus a 32-bit per-lane byte offset
 // in the global address space, so the compiler emits global loads / stores with an SGPR
 // base instead of flat accesses through generic pointers with 64-bit lane addresses.
@@ -132,11 +146,24 @@ __device__ __forceinline__ T gld(const T* base, unsigned i) {
   unsigned off = i * (unsigned)sizeof(T);
   return *(const NZ_GLOBAL T*)((const NZ_GLOBAL char*)base + off);
 }
+// A workgroup-uniform load from memory that no kernel writes while it runs (host-built
+// tables): through the constant address space, so it is a scalar load waited on by its
+// own counter rather than a vector load plus a lane read.
+template <class T>
+__device__ __forceinline__ T cld(const T* base, unsigned i) {
+  return *((const __attribute__((address_space(4))) T*)base + i);
+}
 __device__ __forceinline__ float4 gld(const float4* base, unsigned i) {
   typedef float v4f __attribute__((ext_vector_type(4)));
   unsigned off = i * 16u;
   const v4f v = *(const NZ_GLOBAL v4f*)((const NZ_GLOBAL char*)base + off);
   return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ double2 gld(const double2* base, unsigned i) {
+  typedef double v2d __attribute__((ext_vector_type(2)));
+  unsigned off = i * 16u;
+  const v2d v = *(const NZ_GLOBAL v2d*)((const NZ_GLOBAL char*)base + off);
+  return make_double2(v.x, v.y);
 }
 template <class T>
 __device__ __forceinline__ void gst(T* base, unsigned i, T v) {

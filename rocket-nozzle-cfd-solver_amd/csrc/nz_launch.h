@@ -11,9 +11,12 @@ namespace nzk {
 int tile_shape_count(bool fp64);
 bool tile_shape(bool fp64, int shape, int* tx, int* ty, int* waves = nullptr);
 hipError_t launch_set_inst(DevInst* dst, const DevInst& src, hipStream_t s);
-// with events e0/e1, the kernel's own start and end are recorded into them
+// tilesX0: tile columns of the first instance (sets the XCD chunk); tileInst: instance of
+// each of the ntiles launch tiles (device memory; may be null when ninst is 1). With events
+// e0/e1, the kernel's own start and end are recorded into them.
 hipError_t launch_stage(int stage, bool phys, bool fp64, int shape, const DevInst* d, int ninst, int ntiles,
-                        hipStream_t s, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
+                        int tilesX0, const unsigned short* tileInst, hipStream_t s, hipEvent_t e0 = nullptr,
+                        hipEvent_t e1 = nullptr);
 hipError_t launch_close(const DevInst* d, int ninst, hipStream_t s);
 hipError_t launch_spray(bool fp64, const DevInst* d, int ninst, int nblk, hipStream_t s, hipEvent_t e0 = nullptr,
                         hipEvent_t e1 = nullptr);

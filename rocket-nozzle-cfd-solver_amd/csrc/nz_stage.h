@@ -15,16 +15,6 @@
 namespace nzk {
 
 // ------------------------------------------------------------------------------
-// instance lookup for a flattened tile index (instances sorted by tile0)
-__device__ __forceinline__ int find_inst(const DevInst* __restrict__ I, int n, int t) {
-  int lo = 0, hi = n - 1;
-  while (lo < hi) {
-    const int m = (lo + hi + 1) >> 1;
-    if (I[m].tile0 <= t) lo = m; else hi = m - 1;
-  }
-  return lo;
-}
-
 // Tile of a workgroup. The dispatcher deals workgroups round-robin over the 8 XCDs
 // (workgroup b runs on XCD b % 8), and each XCD has its own L2. The tiles are cut into
 // chunks of `chunk` consecutive tiles (whole tile rows), dealt round-robin to the XCDs:
@@ -47,10 +37,10 @@ __device__ __forceinline__ int xcd_tile(int b, int nblk, int chunk) {
 // FS_STEPINFO (solver.js:404-409): dt from the wave-speed max of the current state,
 // in the arithmetic type of the state
 template <typename R>
-__device__ __forceinline__ void step_info(const Consts& c, const InstState& s, double wmax, R* cur) {
+__device__ __forceinline__ void step_info(const Consts& c, double last0, double last2, double wmax, R* cur) {
   const R w = fmax(R(c.wFloor), R(wmax));
   R dt = fmin(R(c.cfl) / w, R(c.dtMax));
-  const R lastDt = R(s.last[0]), lastT = R(s.last[2]);
+  const R lastDt = R(last0), lastT = R(last2);
   if (lastDt > R(0)) dt = fmin(dt, lastDt * R(1.2f));
   cur[0] = dt;
   cur[1] = lastT;
@@ -80,8 +70,9 @@ __device__ __forceinline__ void fold_issue(const double2* __restrict__ part, uin
     const uint32_t v = (uint32_t)(tid + m * NT);
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
-      const uint32_t k = v + 256u * j;
-      f.p[m][j] = k < n ? part[k] : make_double2(0.0, 0.0);
+      // every lane loads, from an index clamped into the n partials; fold_finish drops the
+      // values of indices past n (selecting here would wait for the loads here)
+      f.p[m][j] = gld(part, min(v + 256u * j, n - 1u));
     }
   }
 }
@@ -94,8 +85,9 @@ __device__ __forceinline__ void fold_finish(const double2* __restrict__ part, ui
     double w = 0.0, r = 0.0;
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
-      w = fmax(w, f.p[m][j].x);
-      r += f.p[m][j].y;
+      const bool in = v + 256u * j < n;
+      w = fmax(w, in ? f.p[m][j].x : 0.0);
+      r += in ? f.p[m][j].y : 0.0;
     }
     for (uint32_t k0 = v + 1024; k0 < n; k0 += 1024) {
       double2 p[4];
@@ -130,12 +122,46 @@ __device__ __forceinline__ void fold_combine(const double* W, const double* R, d
 
 // Stop tests of batched solves after the last completed step: simulated time, residual
 // (after a minimum step count) and step count; 0 while the solve goes on
-__device__ __forceinline__ uint32_t stop_bits(const Consts& c, const InstState& s, double resLast) {
+__device__ __forceinline__ uint32_t stop_bits(const Consts& c, double last2, long long steps, double resLast) {
   uint32_t f = 0u;
-  if (c.tEnd > 0.f && s.last[2] >= (double)c.tEnd) f |= 1u | 4u;
-  if (c.resTol > 0.f && s.steps > 0 && s.steps >= c.minSteps && resLast <= (double)c.resTol) f |= 1u | 2u;
-  if (c.maxSteps > 0 && s.steps >= c.maxSteps) f |= 1u | 8u;
+  if (c.tEnd > 0.f && last2 >= (double)c.tEnd) f |= 1u | 4u;
+  if (c.resTol > 0.f && steps > 0 && steps >= c.minSteps && resLast <= (double)c.resTol) f |= 1u | 2u;
+  if (c.maxSteps > 0 && steps >= c.maxSteps) f |= 1u | 8u;
   return f;
+}
+__device__ __forceinline__ uint32_t stop_bits(const Consts& c, const InstState& s, double resLast) {
+  return stop_bits(c, s.last[2], s.steps, resLast);
+}
+
+// The step-state fields every workgroup of a stage launch reads, loaded together at its
+// start as global loads. The other stage writes them, so no workgroup of this launch
+// changes them. Stage 1: flagsSeen, last, steps, wsnap; stage 2: flags, cur.
+struct StepView {
+  uint32_t flags;
+  double last0, last2;
+  long long steps;
+  unsigned long long wsnap;
+  double cur[4];
+};
+template <int STAGE>
+__device__ __forceinline__ StepView load_step_view(const InstState* st) {
+  const NZ_GLOBAL InstState* g = (const NZ_GLOBAL InstState*)st;
+  StepView v;
+  if (STAGE == 1) {
+    v.flags = g->flagsSeen;
+    v.last0 = g->last[0];
+    v.last2 = g->last[2];
+    v.steps = g->steps;
+    v.wsnap = g->wsnap;
+    v.cur[0] = v.cur[1] = v.cur[2] = v.cur[3] = 0.0;
+  } else {
+    v.flags = g->flags;
+    v.last0 = v.last2 = 0.0;
+    v.steps = 0;
+    v.wsnap = 0ull;
+    for (int q = 0; q < 4; ++q) v.cur[q] = g->cur[q];
+  }
+  return v;
 }
 
 template <int NT>
@@ -357,14 +383,20 @@ struct StageSmem {
 
 // W: minimum waves per SIMD the compiler must fit for stage 1 and the stage 2 kernel
 // without the split step (1: no constraint); the viscous stage 2 is never constrained
+// Launch arguments: the instance table, its length, the XCD chunk (tiles) and, for more
+// than one instance, the instance of every launch tile. Each comes from the kernel
+// arguments (preloaded into SGPRs), so the first memory access of a workgroup is its
+// instance's descriptor, or the one table entry that names it.
 template <typename R, int TX, int TY, int STAGE, bool PHYS, int W>
-__global__ void __launch_bounds__(TX* TY, (STAGE == 2 && PHYS) ? 1 : W) k_stage(const DevInst* __restrict__ insts, int ninst) {
+__global__ void __launch_bounds__(TX* TY, (STAGE == 2 && PHYS) ? 1 : W)
+    k_stage(const DevInst* __restrict__ insts, int ninst, int chunk, const unsigned short* __restrict__ tileInst) {
   constexpr int NT = TX * TY;
   using S = StageSmem<R, TX, TY, STAGE == 2 && PHYS>;
   __shared__ S sm;
   const int tid = threadIdx.x;
-  const int t = xcd_tile(blockIdx.x, gridDim.x, max(insts[0].tilesX * NZ_XCD_CHUNK_ROWS, 1));
-  const int ii = find_inst(insts, ninst, t);
+  const int t = xcd_tile(blockIdx.x, gridDim.x, chunk);
+  const int ii = ninst == 1 ? 0 : (int)tileInst[t];
+  touch_desc(insts + ii);
   const DevInst& I = insts[ii];
   const Consts& c = I.c;
   InstState* st = I.st;
@@ -391,13 +423,14 @@ __global__ void __launch_bounds__(TX* TY, (STAGE == 2 && PHYS) ? 1 : W) k_stage(
   // the last step's partials: one per tile of this tiling (see FUSED_CLOSE_MAX)
   const uint32_t nPart = (uint32_t)(I.tilesX * I.tilesY);
   const bool fused = nPart <= FUSED_CLOSE_MAX;
+  StepView sv;
   auto step_state = [&]() {
-    was = !((STAGE == 1 ? st->flagsSeen : st->flags) & 1u);
+    was = !(sv.flags & 1u);
     active = was;
     if (STAGE == 1) {
-      tsel = R(st->last[2]);
+      tsel = R(sv.last2);
       if (was) {
-        const uint32_t f = stop_bits(c, *st, INFINITY);
+        const uint32_t f = stop_bits(c, sv.last2, sv.steps, INFINITY);
         if (f) {
           active = false;
           if (lt_ == 0) {
@@ -418,7 +451,7 @@ __global__ void __launch_bounds__(TX* TY, (STAGE == 2 && PHYS) ? 1 : W) k_stage(
         }
       }
     } else {
-      cur[0] = R(st->cur[0]); cur[1] = R(st->cur[1]); cur[2] = R(st->cur[2]); cur[3] = R(st->cur[3]);
+      cur[0] = R(sv.cur[0]); cur[1] = R(sv.cur[1]); cur[2] = R(sv.cur[2]); cur[3] = R(sv.cur[3]);
       dt = cur[0];
       tsel = cur[2];
       if (lt_ == 0 && tid == 0) st->flagsSeen = st->flags;
@@ -434,7 +467,10 @@ __global__ void __launch_bounds__(TX* TY, (STAGE == 2 && PHYS) ? 1 : W) k_stage(
   };
   // Throughput-tuned shapes (W = 4, many workgroups per CU in turn) read the step state
   // first; the latency-tuned shapes issue the tile's loads first (below).
-  if constexpr (W >= 4) step_state();
+  if constexpr (W >= 4) {
+    sv = load_step_view<STAGE>(st);
+    step_state();
+  }
 
   // ---- 1. primitive tile with ghost halo (FS_PRIM); this tile's extra faces. The tile's
   // loads depend only on the tile; on the latency-tuned shapes they are issued ahead of
@@ -446,13 +482,18 @@ __global__ void __launch_bounds__(TX* TY, (STAGE == 2 && PHYS) ? 1 : W) k_stage(
   const int ci = i0 + tx, cj = j0 + ty;
   const bool inDom = ci < Nx && cj < Nr;
   const long long k = (long long)cj * Nx + ci;
+  // Every load below is issued by every lane, from an address clamped into its array, and
+  // lanes with nothing to load discard the value: loads under a divergent branch would be
+  // waited for inside it, one round trip per branch instead of one for the whole set.
   uint32_t code = C_IS_SLAVE;
   float4 gg = make_float4(0.f, 0.f, 0.f, 0.f);
   R a[4], b[4];
-  if (inDom) {
-    code = gld(I.code, (unsigned)k);
-    gg = gld(I.gg, (unsigned)k);
-    for (int q = 0; q < 4; ++q) { a[q] = gld(Uin + q * Ns, (unsigned)k); b[q] = gld(Uin + (4 + q) * Ns, (unsigned)k); }
+  {
+    const unsigned kc = (unsigned)(min(cj, Nr - 1) * Nx + min(ci, Nx - 1));
+    const uint32_t cd = gld(I.code, kc);
+    const float4 g4 = gld(I.gg, kc);
+    for (int q = 0; q < 4; ++q) { a[q] = gld(Uin + q * Ns, kc); b[q] = gld(Uin + (4 + q) * Ns, kc); }
+    if (inDom) { code = cd; gg = g4; }
   }
   // eddy-viscosity length scales of this thread's cells of the viscous ring tile
   constexpr int LPT = (S::TW * S::THh + NT - 1) / NT;
@@ -468,36 +509,51 @@ __global__ void __launch_bounds__(TX* TY, (STAGE == 2 && PHYS) ? 1 : W) k_stage(
   // this tile's extra faces (faces of its merged cells outside the tile) and the far
   // cells they reach: workgroup-uniform index loads, issued before the tile's own loads
   // so that their latency is not added after them
-  const int exBase = I.exStart[lt_];
-  const int nEx = min(I.exStart[lt_ + 1] - exBase, S::MAXE);
-  const int fcBase = I.fcStart[lt_];
-  const int nF = min(I.fcStart[lt_ + 1] - fcBase, S::MAXF);
+  const int exBase = cld(I.exStart, (unsigned)lt_);
+  const int nEx = min(cld(I.exStart, (unsigned)lt_ + 1u) - exBase, S::MAXE);
+  const int fcBase = cld(I.fcStart, (unsigned)lt_);
+  const int nF = min(cld(I.fcStart, (unsigned)lt_ + 1u) - fcBase, S::MAXF);
   // every load of this thread's cells and face areas is issued before any is used
   constexpr int CPT = (S::PW * S::PH + NT - 1) / NT, FPT = (S::NFX + S::NFR + NT - 1) / NT;
   CellRaw<R> raw[CPT];
   float area[FPT];
 #pragma unroll
   for (int r = 0; r < CPT; ++r) {
-    const int idx = tid + r * NT;
-    if (idx < S::PW * S::PH) raw[r] = cell_src_load<R>(I, Uin, i0 - 2 + idx % S::PW, j0 - 2 + idx / S::PW);
+    const int idx = min(tid + r * NT, S::PW * S::PH - 1);  // lanes past the halo reload its last cell
+    raw[r] = cell_src_load<R>(I, Uin, i0 - 2 + idx % S::PW, j0 - 2 + idx / S::PW);
   }
 #pragma unroll
   for (int r = 0; r < FPT; ++r) {
-    const int f = tid + r * NT;
-    area[r] = 0.f;
-    if (f < S::NFX + S::NFR) {
-      const bool dirR = f >= S::NFX;
-      const int g = dirR ? f - S::NFX : f;
-      const int fi = i0 + (dirR ? g % TX : g % (TX + 1)), fj = j0 + (dirR ? g / TX : g / (TX + 1));
-      if (dirR ? (fi < Nx && fj <= Nr) : (fi <= Nx && fj < Nr)) area[r] = face_area(I, dirR, fi, fj);
-    }
+    const int f = min(tid + r * NT, S::NFX + S::NFR - 1);
+    const bool dirR = f >= S::NFX;
+    const int g = dirR ? f - S::NFX : f;
+    const int fi = i0 + (dirR ? g % TX : g % (TX + 1)), fj = j0 + (dirR ? g / TX : g / (TX + 1));
+    // x faces: fi <= Nx, fj < Nr; r faces: fi < Nx, fj <= Nr (0 outside)
+    const bool in = dirR ? (fi < Nx && fj <= Nr) : (fi <= Nx && fj < Nr);
+    // one load from a per-lane base (no branch on the direction); both bases are taken as
+    // workgroup-uniform values first, so the choice is a register select, not a per-lane
+    // load of the descriptor field
+    const float* pax = I.ax;
+    const float* par = I.ar;
+    asm("" : "+s"(pax));
+    asm("" : "+s"(par));
+    const float* base = dirR ? par : pax;
+    const int row = dirR ? Nx : Nx + 1;
+    const float ld = gld(base, (unsigned)(min(fj, dirR ? Nr : Nr - 1) * row + min(fi, dirR ? Nx - 1 : Nx)));
+    area[r] = (in && tid + r * NT < S::NFX + S::NFR) ? ld : 0.f;
   }
   // the last step's partials (stage 1): loaded with the tile, folded once the tile's
   // primitives are in LDS, so that their round trip overlaps the tile's
   // (tiles of fewer than 256 threads hold more partials per thread: they load them late)
   constexpr bool EARLY_FOLD = NT >= 256;
   FoldLoads<NT> fl;
-  if (STAGE == 1 && EARLY_FOLD && fused) fold_issue<NT>(I.part, nPart, tid, fl);
+  if constexpr (W < 4) sv = load_step_view<STAGE>(st);
+  // (issued whether or not this launch folds them: a branch here would take the first
+  // use of the loads, and its wait, into it)
+  if (STAGE == 1 && EARLY_FOLD) fold_issue<NT>(I.part, nPart, tid, fl);
+  // every load above is issued before any of them is waited for: nothing below may be
+  // moved above this point, and no load above below it
+  asm volatile("" ::: "memory");
   if constexpr (W < 4) step_state();
 
   double wloc = 0.0;
@@ -580,8 +636,8 @@ __global__ void __launch_bounds__(TX* TY, (STAGE == 2 && PHYS) ? 1 : W) k_stage(
     if (STAGE == 1) {
       double wF = 0.0, rF = 0.0;
       if (fused) fold_combine(sm.foldW, sm.foldR, wF, rF);
-      const double wmax = fused ? wF : __longlong_as_double((long long)st->wsnap);
-      step_info<R>(c, *st, wmax, cur);
+      const double wmax = fused ? wF : __longlong_as_double((long long)sv.wsnap);
+      step_info<R>(c, sv.last0, sv.last2, wmax, cur);
       dt = cur[0];
       if (lt_ == 0 && tid == 0) {
         st->cur[0] = cur[0]; st->cur[1] = cur[1]; st->cur[2] = cur[2]; st->cur[3] = cur[3];

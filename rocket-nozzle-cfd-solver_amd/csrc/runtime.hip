@@ -210,13 +210,13 @@ int nz_set_state(nz_solver* s, const float* A, const float* B) {
 static int enqueue_step(nz_solver* s, bool phys, bool spray, hipEvent_t* ev) {
   const bool fp64 = s->prec == 64;
   const int ntiles = s->I.tilesX * s->I.tilesY;
-  HIPCHK(nzk::launch_stage(1, false, fp64, s->I.shape, s->dinst, 1, ntiles, s->stream, ev ? ev[0] : nullptr,
-                           ev ? ev[1] : nullptr), "stage 1");
+  HIPCHK(nzk::launch_stage(1, false, fp64, s->I.shape, s->dinst, 1, ntiles, s->I.tilesX, nullptr, s->stream,
+                           ev ? ev[0] : nullptr, ev ? ev[1] : nullptr), "stage 1");
   if (spray)
     HIPCHK(nzk::launch_spray(fp64, s->dinst, 1, s->h.npblk, s->stream, ev ? ev[2] : nullptr, ev ? ev[3] : nullptr),
            "spray");
-  HIPCHK(nzk::launch_stage(2, phys, fp64, s->I.shape, s->dinst, 1, ntiles, s->stream, ev ? ev[4] : nullptr,
-                           ev ? ev[5] : nullptr), "stage 2");
+  HIPCHK(nzk::launch_stage(2, phys, fp64, s->I.shape, s->dinst, 1, ntiles, s->I.tilesX, nullptr, s->stream,
+                           ev ? ev[4] : nullptr, ev ? ev[5] : nullptr), "stage 2");
   // more partials than the next stage 1 folds itself: fold them now
   if ((unsigned)ntiles > nzk::FUSED_CLOSE_MAX) HIPCHK(nzk::launch_close(s->dinst, 1, s->stream), "close");
   return 0;

@@ -18,21 +18,22 @@ namespace nzk {
 namespace {
 // with events, the kernel's own start / end are recorded (the dispatch packet carries them)
 template <class K>
-hipError_t go(K k, dim3 g, dim3 b, hipStream_t s, hipEvent_t e0, hipEvent_t e1, const DevInst* d, int ninst) {
-  if (e0 && e1) hipExtLaunchKernelGGL(k, g, b, 0, s, e0, e1, 0, d, ninst);
-  else hipLaunchKernelGGL(k, g, b, 0, s, d, ninst);
+hipError_t go(K k, dim3 g, dim3 b, hipStream_t s, hipEvent_t e0, hipEvent_t e1, const DevInst* d, int ninst, int chunk,
+              const unsigned short* ti) {
+  if (e0 && e1) hipExtLaunchKernelGGL(k, g, b, 0, s, e0, e1, 0, d, ninst, chunk, ti);
+  else hipLaunchKernelGGL(k, g, b, 0, s, d, ninst, chunk, ti);
   return hipGetLastError();
 }
 }  // namespace
 
-hipError_t NZ_CAT(stage_launch_, NZ_STAGE_NAME)(int stage, bool phys, const DevInst* d, int ninst, int ntiles,
-                                                 hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
+hipError_t NZ_CAT(stage_launch_, NZ_STAGE_NAME)(int stage, bool phys, const DevInst* d, int ninst, int ntiles, int chunk,
+                                                 const unsigned short* ti, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
   using R = NZ_STAGE_R;
   constexpr int TX = NZ_STAGE_TX, TY = NZ_STAGE_TY, W = NZ_STAGE_W;
   const dim3 g(ntiles), b(TX * TY);
-  if (stage == 1) return go(k_stage<R, TX, TY, 1, false, W>, g, b, s, e0, e1, d, ninst);
-  if (phys) return go(k_stage<R, TX, TY, 2, true, W>, g, b, s, e0, e1, d, ninst);
-  return go(k_stage<R, TX, TY, 2, false, W>, g, b, s, e0, e1, d, ninst);
+  if (stage == 1) return go(k_stage<R, TX, TY, 1, false, W>, g, b, s, e0, e1, d, ninst, chunk, ti);
+  if (phys) return go(k_stage<R, TX, TY, 2, true, W>, g, b, s, e0, e1, d, ninst, chunk, ti);
+  return go(k_stage<R, TX, TY, 2, false, W>, g, b, s, e0, e1, d, ninst, chunk, ti);
 }
 
 }  // namespace nzk

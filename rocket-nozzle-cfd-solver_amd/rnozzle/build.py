@@ -25,9 +25,9 @@ CPP_SOURCES = ["setup.cpp", "errors.cpp"]
 # per-thread arrays stay in private memory rather than being moved into LDS, whose size
 # sets how many stage workgroups share a CU
 DEVICE_FLAGS = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-mllvm", "-disable-promote-alloca-to-lds"]
-# the stage kernels' arguments (instance table pointer, instance count) preloaded into
+# the stage kernels' arguments (instance table, instance count, XCD chunk, tile table) preloaded into
 # SGPRs at wave launch, so the first table read does not wait on a kernel-argument load
-STAGE_FLAGS = ["-mllvm", "-amdgpu-kernarg-preload-count=2"]
+STAGE_FLAGS = ["-mllvm", "-amdgpu-kernarg-preload-count=4"]
 
 
 def stage_shapes():
