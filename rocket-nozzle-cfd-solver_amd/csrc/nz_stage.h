@@ -830,6 +830,33 @@ __global__ void __launch_bounds__(TX* TY, (STAGE == 2 && PHYS) ? 1 : W)
     // ---- 4. viscous / wall / afterburning split step (FS_VCELL, FS_VFLUX, FS_SOURCE)
     if constexpr (STAGE == 2 && PHYS) {
     if (c.visc || c.chem) {  // per instance: a batch mixes both kinds
+      // Zukoski separation (below) reads the stage-1 state a few cells downstream of a
+      // supersonic boundary-layer host: waves holding a boundary-layer host issue those
+      // loads here and fold them after the viscous cells, so their latency overlaps that
+      // work. zSup: the host is supersonic; zup, zMn: its speed and Mach number.
+      const bool zHost = c.visc && host && (code & C_BL);
+      const bool zWave = __ballot(zHost) != 0ull;  // uniform over the wave
+      bool zSup = false;
+      R zup = R(0), zMn = R(0), zpd = R(0);
+      uint32_t zc[4];
+      R zq[4][5];
+      if (zWave) {
+        zup = sq(qS.u * qS.u + qS.v * qS.v);
+        zMn = dv(zup, sq(dv((R(1) + rc(qS.G)) * qS.p, qS.r)));
+        zSup = zHost && zMn > R(1);
+        const R iup = rc(zup);
+        const R shx = zSup ? qS.u * iup : R(0), shy = zSup ? qS.v * iup : R(0);
+#pragma unroll
+        for (int z = 0; z < 4; ++z) {
+          const int n = z < 2 ? 4 : 6, m = z & 1;
+          const int Dx = min(max(ci + (int)round(R(n) * shx), 0), Nx - 1);
+          const int Dy = min(max(cj + (int)floor(R(n) * shy) + m, 0), Nr - 1);
+          const unsigned kd = (unsigned)(Dy * Nx + Dx);
+          zc[z] = gld(I.code, kd);
+          for (int q = 0; q < 4; ++q) zq[z][q] = gld(U1 + q * Ns, kd);
+          zq[z][4] = gld(U1 + 6 * Ns, kd);
+        }
+      }
       __syncthreads();  // inviscid fluxes consumed; reuse their LDS
 #pragma unroll
       for (int r = 0; r < LPT; ++r) {
@@ -846,6 +873,13 @@ __global__ void __launch_bounds__(TX* TY, (STAGE == 2 && PHYS) ? 1 : W)
         for (int q = 0; q < 4; ++q) { sm.u.vis.T[q][rj][ri] = T1[q]; sm.u.vis.T[4 + q][rj][ri] = T2[q]; }
       }
       __syncthreads();
+      // the probes' largest pressure over the cells that are not solid, in probe order
+      if (zWave) {
+#pragma unroll
+        for (int z = 0; z < 4; ++z)
+          if (!(zc[z] & C_SOLID))
+            zpd = fmax(zpd, to_prim<R>(zq[z][0], zq[z][1], zq[z][2], zq[z][3], R(0), R(0), zq[z][4], R(0)).p);
+      }
       // per-cell T1/T2: the LDS ring tile, else computed from primitives
       auto anyT = [&](int gi, int gj, R* T1, R* T2) {
         const int ri = gi - (i0 - 1), rj = gj - (j0 - 1);
@@ -964,22 +998,11 @@ __global__ void __launch_bounds__(TX* TY, (STAGE == 2 && PHYS) ? 1 : W)
           }
           // Zukoski separation in the boundary-layer band (solver.js:613-630)
           if (code & C_BL) {
-            const R up = sq(velx * velx + vely * vely), Mn = dv(up, sq(dv((R(1) + rc(G)) * p, rho)));
-            if (Mn > R(1)) {
-              const R iup = rc(up);
-              const R shx = velx * iup, shy = vely * iup;
-              R pd = R(0);
-              for (int n = 4; n <= 6; n += 2)
-                for (int m = 0; m <= 1; ++m) {
-                  int Dx = ci + (int)round(R(n) * shx);
-                  int Dy = cj + (int)floor(R(n) * shy) + m;
-                  Dx = min(max(Dx, 0), Nx - 1);
-                  Dy = min(max(Dy, 0), Nr - 1);
-                  const long long kd = (long long)Dy * Nx + Dx;
-                  if (!(gld(I.code, (unsigned)kd) & C_SOLID)) pd = fmax(pd, load_prim<R>(U1, Ns, kd).p);
-                }
+            // probes at n = 4, 6 along the flow, rows floor(n shy) + 0, 1: loaded and folded above
+            const R up = zup, Mn = zMn;
+            if (zSup) {
               const R crit = R(1) + R(0.5) * Mn;
-              const R fs = smoothstepR(crit, R(1.2) * crit, dv(pd, p)) * V * rho * up;
+              const R fs = smoothstepR(crit, R(1.2) * crit, dv(zpd, p)) * V * rho * up;
               L[1] -= fs * velx;
               L[2] -= fs * vely;
             } else if (velx > R(0) && p < R(1.3) * R(c.chemK[3])) {
