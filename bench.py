@@ -183,6 +183,7 @@ def bench_batch(args):
                    "grids": sorted({f"{s.Nx}x{s.Nr}" for s in setups})},
         "wall_s": wall, "cell_updates_per_s": cu / wall, "setup_build_s": t_setup, "stops": stops,
         "latency_s": {"p50": pct(lat, 50), "p99": pct(lat, 99), "max": max(lat)},
+        "gpu_busy_fraction": b.gpu_busy() / wall,
     }
     if args.plain != 0:
         n = len(ids) if args.plain < 0 else min(args.plain, len(ids))
@@ -265,6 +266,8 @@ def bench_serve(args):
         "wall_s": wall,
         "time_to_converged_s": {"p50": pct(lat, 50), "p99": pct(lat, 99),
                                 "nozzle_p50": pct(lat_noz, 50), "injector_p50": pct(lat_inj, 50)},
+        # device time of the run chunks over the wall time from the first arrival to the last result
+        "gpu_busy_fraction": b.gpu_busy() / wall,
     }
     return out
 
@@ -304,7 +307,7 @@ def main():
     ap.add_argument("--seed", type=int, default=20240501)
     ap.add_argument("--chunk", type=int, default=200, help="batch/serve: steps per run call")
     ap.add_argument("--plain", type=int, default=-1, help="batch: solves re-run one at a time (-1 all, 0 none)")
-    ap.add_argument("--budget-gb", type=float, default=64.0)
+    ap.add_argument("--budget-gb", type=float, default=16.0)
     ap.add_argument("--rate", type=float, default=200.0, help="serve: mean arrivals per second")
     ap.add_argument("--requests", type=int, default=64)
     ap.add_argument("--policy", default="sjf", choices=["sjf", "fifo"], help="serve: admission order")
@@ -421,18 +424,16 @@ def main():
         # the batched and serving workloads beside the headline (their own timed regions)
         print("[bench] batch: 64 solves batched, then one at a time", file=sys.stderr, flush=True)
         bt = bench_batch(args)
-        out["batch"] = {k: bt[k] for k in ("value", "unit", "wall_s", "cell_updates_per_s", "stops", "latency_s")}
+        out["batch"] = {k: bt[k] for k in ("value", "unit", "wall_s", "cell_updates_per_s", "stops", "latency_s",
+                                           "gpu_busy_fraction")}
         out["batch"]["config"] = bt["config"]
         if "plain" in bt:
             out["batch"]["plain"] = bt["plain"]
             out["batch"]["speedup_vs_plain"] = bt["speedup_vs_plain"]
         print("[bench] serve: Poisson arrivals", file=sys.stderr, flush=True)
         sv = bench_serve(args)
-        out["serve"] = {k: sv[k] for k in ("value", "unit", "wall_s", "time_to_converged_s")}
+        out["serve"] = {k: sv[k] for k in ("value", "unit", "wall_s", "time_to_converged_s", "gpu_busy_fraction")}
         out["serve"]["config"] = sv["config"]
-        for k in ("gpu_busy",):
-            if k in sv:
-                out["serve"][k] = sv[k]
     print(json.dumps(out), flush=True)
 
 

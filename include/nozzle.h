@@ -231,8 +231,9 @@ int nz_integrals_read(nz_solver* s, const nz_params* P, nz_integrals* out);
 /* else is resident (bit-identical, the spray included: its deposit sums   */
 /* fixed-point integers, whose order leaves no trace). Jobs are admitted    */
 /* first-fit in submission order while the device-memory budget and the     */
-/* resident limit allow; a solve retires when it meets its stop test, its   */
-/* memory is freed and queued jobs take its place at the next run call.     */
+/* resident limit allow; their buffers come from one device slab of the     */
+/* budget allocated at nz_batch_create. A solve retires when it meets its   */
+/* stop test; its memory returns to the slab and queued jobs take its place.*/
 /* ------------------------------------------------------------------------ */
 typedef struct nz_batch nz_batch;
 typedef struct nz_batch_job {
@@ -269,10 +270,20 @@ uint64_t nz_batch_instance_bytes(int Nx, int Nr, int precision, int parcels);
 /* copy a job into the queue; *id identifies it from then on. Fails if the job
  * alone exceeds the budget. */
 int nz_batch_submit(nz_batch* b, const nz_batch_job* job, int64_t* id);
-/* admit what fits, advance every resident `steps` steps (a captured graph
- * when enabled), then retire the solves that stopped. Returns after the
- * device has finished; *resident / *queued (may be NULL) count what is left. */
+/* admit what fits and enqueue `steps` steps of every resident (a captured graph
+ * when enabled); then wait for the chunk the previous call enqueued, read its
+ * stop flags and start retiring the solves that stopped in it. The device works
+ * on this call's chunk while the host retires and admits, so a run loop keeps it
+ * busy. A retiring solve's integrals and kept state are collected at a later call
+ * (at once when nothing is left to run), when it is listed by nz_batch_finished.
+ * steps = 0: admit only (no waiting, no filing: nz_batch_finished still lists
+ * the last stepping call's jobs). *resident counts
+ * the solves not yet filed (running or retiring), *queued the waiting jobs (both
+ * may be NULL); a loop of run calls ends when both are 0. */
 int nz_batch_run(nz_batch* b, int steps, int* resident, int* queued);
+/* device seconds of the chunks whose stop flags have been read (event-timed),
+ * for the busy fraction of a run loop: busy / wall time */
+int nz_batch_gpu_busy(const nz_batch* b, double* seconds);
 int nz_batch_result_get(nz_batch* b, int64_t id, nz_batch_result* out);
 /* state of a running solve, or of a finished one submitted with keep_state */
 int nz_batch_read_state(nz_batch* b, int64_t id, float* A, float* B);
@@ -289,7 +300,7 @@ int nz_batch_set_tile_shape(nz_batch* b, int shape);
  * during the trials (it is real work); stop tests still apply and the stopped solves
  * retire at the next nz_batch_run. us_per_step: NULL or >= nz_tile_shapes() entries. */
 int nz_batch_autotune(nz_batch* b, int steps, int* best, double* us_per_step);
-/* ids of the jobs that finished in the last nz_batch_run; returns the count
+/* ids of the jobs filed as finished by the last nz_batch_run with steps > 0; returns the count
  * (at most cap are written) */
 int nz_batch_finished(const nz_batch* b, int64_t* ids, int cap);
 

@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <iterator>
 
 #include "nz_inst.h"
 #include "nz_launch.h"
@@ -16,14 +17,83 @@ using nzk::Consts;
 using nzk::DevInst;
 using nzk::InstState;
 
+int DevArena::create(size_t bytes) {
+  destroy();
+  bytes = (bytes + 255) / 256 * 256;
+  HIPCHK(hipMalloc(&base_, bytes), "device slab");
+  size_ = bytes;
+  used_ = 0;
+  free_[0] = bytes;
+  return 0;
+}
+
+void DevArena::destroy() {
+  if (base_) (void)hipFree(base_);
+  base_ = nullptr;
+  size_ = used_ = 0;
+  free_.clear();
+  live_.clear();
+}
+
+void* DevArena::alloc(size_t bytes) {
+  bytes = std::max<size_t>((bytes + 255) / 256 * 256, 256);
+  for (auto it = free_.begin(); it != free_.end(); ++it) {
+    if (it->second < bytes) continue;
+    const size_t off = it->first, rest = it->second - bytes;
+    free_.erase(it);
+    if (rest) free_[off + bytes] = rest;
+    live_[off] = bytes;
+    used_ += bytes;
+    return base_ + off;
+  }
+  failed_ = true;
+  return nullptr;
+}
+
+void DevArena::release(void* p) {
+  if (!p) return;
+  const size_t off = (size_t)(static_cast<char*>(p) - base_);
+  auto it = live_.find(off);
+  if (it == live_.end()) return;
+  size_t start = off, bytes = it->second;
+  live_.erase(it);
+  used_ -= bytes;
+  auto next = free_.lower_bound(start);
+  if (next != free_.end() && next->first == start + bytes) {
+    bytes += next->second;
+    next = free_.erase(next);
+  }
+  if (next != free_.begin()) {
+    auto prev = std::prev(next);
+    if (prev->first + prev->second == start) {
+      start = prev->first;
+      bytes += prev->second;
+      free_.erase(prev);
+    }
+  }
+  free_[start] = bytes;
+}
+
+hipError_t inst_dalloc(Inst& I, void** p, size_t bytes) {
+  if (!I.arena) return hipMalloc(p, bytes);
+  *p = I.arena->alloc(bytes);
+  return *p ? hipSuccess : hipErrorOutOfMemory;
+}
+
+void inst_dfree(Inst& I, void* p) {
+  if (!p) return;
+  if (I.arena) I.arena->release(p);
+  else (void)hipFree(p);
+}
+
 namespace {
 template <class T>
-void hfree(T*& p) {
-  if (p) (void)hipFree(p);
-  p = nullptr;
+hipError_t dalloc(Inst& I, T** p, size_t bytes) {
+  return inst_dalloc(I, reinterpret_cast<void**>(p), bytes);
 }
-void vfree(void*& p) {
-  if (p) (void)hipFree(p);
+template <class T>
+void dfree(Inst& I, T*& p) {
+  inst_dfree(I, (void*)p);
   p = nullptr;
 }
 }  // namespace
@@ -52,8 +122,9 @@ size_t inst_bytes(int Nx, int Nr, int prec, int NP) {
          + tiles * (16 + 8) + (size_t)Nx * 32 + 4096;     // partials, starts, wall row, small
 }
 
-int inst_alloc(Inst& I, int Nx, int Nr, int prec, InstState* st, int shape) {
+int inst_alloc(Inst& I, int Nx, int Nr, int prec, InstState* st, int shape, DevArena* arena) {
   inst_free(I);
+  I.arena = arena;
   if (!nzk::tile_shape(prec == 64, shape, &I.TX, &I.TY)) return nz_fail(-1, "unknown tile shape");
   I.shape = shape;
   I.prec = prec;
@@ -65,33 +136,33 @@ int inst_alloc(Inst& I, int Nx, int Nr, int prec, InstState* st, int shape) {
   I.tilesX = (Nx + I.TX - 1) / I.TX;
   I.tilesY = (Nr + I.TY - 1) / I.TY;
   const size_t tiles = (size_t)I.tilesX * I.tilesY, R = I.rsz;
-  HIPCHK(hipMalloc(&I.U0, (size_t)I.Ns * 8 * R), "hipMalloc state");
-  HIPCHK(hipMalloc(&I.U1, (size_t)I.Ns * 8 * R), "hipMalloc state");
+  HIPCHK(dalloc(I, &I.U0, (size_t)I.Ns * 8 * R), "allocate state");
+  HIPCHK(dalloc(I, &I.U1, (size_t)I.Ns * 8 * R), "allocate state");
   HIPCHK(hipMemset(I.U0, 0, (size_t)I.Ns * 8 * R), "memset");
   HIPCHK(hipMemset(I.U1, 0, (size_t)I.Ns * 8 * R), "memset");
-  HIPCHK(hipMalloc(&I.ax, (size_t)Nr * (Nx + 1) * 4), "hipMalloc ax");
-  HIPCHK(hipMalloc(&I.ar, (size_t)(Nr + 1) * Nx * 4), "hipMalloc ar");
-  HIPCHK(hipMalloc(&I.gg, (size_t)N * 16), "hipMalloc gg");
-  HIPCHK(hipMalloc(&I.code, (size_t)N * 4), "hipMalloc code");
-  HIPCHK(hipMalloc(&I.lt, (size_t)N * 4), "hipMalloc lt");
-  HIPCHK(hipMalloc(&I.jw, (size_t)Nx * 16), "hipMalloc jWall");
-  HIPCHK(hipMalloc(&I.src, (size_t)I.Ns * 32), "hipMalloc src");
-  HIPCHK(hipMalloc(&I.tab, 256 * 3 * 16), "hipMalloc tab");
+  HIPCHK(dalloc(I, &I.ax, (size_t)Nr * (Nx + 1) * 4), "allocate ax");
+  HIPCHK(dalloc(I, &I.ar, (size_t)(Nr + 1) * Nx * 4), "allocate ar");
+  HIPCHK(dalloc(I, &I.gg, (size_t)N * 16), "allocate gg");
+  HIPCHK(dalloc(I, &I.code, (size_t)N * 4), "allocate code");
+  HIPCHK(dalloc(I, &I.lt, (size_t)N * 4), "allocate lt");
+  HIPCHK(dalloc(I, &I.jw, (size_t)Nx * 16), "allocate jWall");
+  HIPCHK(dalloc(I, &I.src, (size_t)I.Ns * 32), "allocate src");
+  HIPCHK(dalloc(I, &I.tab, 256 * 3 * 16), "allocate tab");
   I.partCap = part_capacity(Nx, Nr, prec);
-  HIPCHK(hipMalloc(&I.part, I.partCap * sizeof(double2)), "hipMalloc partials");
+  HIPCHK(dalloc(I, &I.part, I.partCap * sizeof(double2)), "allocate partials");
   HIPCHK(hipMemset(I.part, 0, I.partCap * sizeof(double2)), "memset");
-  HIPCHK(hipMalloc(&I.exStart, (tiles + 1) * sizeof(int)), "hipMalloc extra starts");
-  HIPCHK(hipMalloc(&I.fcStart, (tiles + 1) * sizeof(int)), "hipMalloc far starts");
-  HIPCHK(hipMalloc(&I.exFaces, sizeof(uint2)), "hipMalloc extra faces");
-  HIPCHK(hipMalloc(&I.fcCells, sizeof(int2)), "hipMalloc far cells");
-  HIPCHK(hipMalloc(&I.exFlux, 16 * 8), "hipMalloc extra fluxes");
-  HIPCHK(hipMalloc(&I.wallrow, (size_t)Nx * 16), "hipMalloc wall row");
-  HIPCHK(hipMalloc(&I.dint, (16 + 2 * (size_t)nzk::INTEG_BLOCKS_MAX) * 8), "hipMalloc integrals");
+  HIPCHK(dalloc(I, &I.exStart, (tiles + 1) * sizeof(int)), "allocate extra starts");
+  HIPCHK(dalloc(I, &I.fcStart, (tiles + 1) * sizeof(int)), "allocate far starts");
+  HIPCHK(dalloc(I, &I.exFaces, sizeof(uint2)), "allocate extra faces");
+  HIPCHK(dalloc(I, &I.fcCells, sizeof(int2)), "allocate far cells");
+  HIPCHK(dalloc(I, &I.exFlux, 16 * 8), "allocate extra fluxes");
+  HIPCHK(dalloc(I, &I.wallrow, (size_t)Nx * 16), "allocate wall row");
+  HIPCHK(dalloc(I, &I.dint, (16 + 2 * (size_t)nzk::INTEG_BLOCKS_MAX) * 8), "allocate integrals");
   if (st) {
     I.st = st;
     I.ownSt = false;
   } else {
-    HIPCHK(hipMalloc(&I.st, sizeof(InstState)), "hipMalloc state info");
+    HIPCHK(dalloc(I, &I.st, sizeof(InstState)), "allocate state info");
     I.ownSt = true;
   }
   // empty geometry (all faces closed, every cell its own host) until setGeometry
@@ -118,11 +189,11 @@ int inst_alloc(Inst& I, int Nx, int Nr, int prec, InstState* st, int shape) {
 }
 
 void inst_free(Inst& I) {
-  vfree(I.U0); vfree(I.U1);
-  hfree(I.ax); hfree(I.ar); hfree(I.lt); hfree(I.jw); hfree(I.src); hfree(I.parc); hfree(I.tab);
-  hfree(I.gg); hfree(I.code); hfree(I.exStart); hfree(I.exFaces); hfree(I.fcStart); hfree(I.fcCells);
-  vfree(I.exFlux); hfree(I.part); hfree(I.wallrow); hfree(I.dint);
-  if (I.ownSt) hfree(I.st);
+  dfree(I, I.U0); dfree(I, I.U1);
+  dfree(I, I.ax); dfree(I, I.ar); dfree(I, I.lt); dfree(I, I.jw); dfree(I, I.src); dfree(I, I.parc); dfree(I, I.tab);
+  dfree(I, I.gg); dfree(I, I.code); dfree(I, I.exStart); dfree(I, I.exFaces); dfree(I, I.fcStart); dfree(I, I.fcCells);
+  dfree(I, I.exFlux); dfree(I, I.part); dfree(I, I.wallrow); dfree(I, I.dint);
+  if (I.ownSt) dfree(I, I.st);
   I.st = nullptr;
   I.haveGeo = false;
 }
@@ -214,12 +285,12 @@ static int build_extra_faces(Inst& I, std::vector<uint32_t>& code) {
   }
   start[ntiles] = (int)faces.size();
   fstart[ntiles] = (int)cells.size();
-  hfree(I.exFaces);
-  hfree(I.fcCells);
-  vfree(I.exFlux);
-  HIPCHK(hipMalloc(&I.exFaces, std::max<size_t>(faces.size(), 1) * sizeof(uint2)), "hipMalloc extra faces");
-  HIPCHK(hipMalloc(&I.fcCells, std::max<size_t>(cells.size(), 1) * sizeof(int2)), "hipMalloc far cells");
-  HIPCHK(hipMalloc(&I.exFlux, std::max<size_t>(faces.size(), 1) * 16 * I.rsz), "hipMalloc extra fluxes");
+  dfree(I, I.exFaces);
+  dfree(I, I.fcCells);
+  dfree(I, I.exFlux);
+  HIPCHK(dalloc(I, &I.exFaces, std::max<size_t>(faces.size(), 1) * sizeof(uint2)), "allocate extra faces");
+  HIPCHK(dalloc(I, &I.fcCells, std::max<size_t>(cells.size(), 1) * sizeof(int2)), "allocate far cells");
+  HIPCHK(dalloc(I, &I.exFlux, std::max<size_t>(faces.size(), 1) * 16 * I.rsz), "allocate extra fluxes");
   if (!faces.empty()) HIPCHK(hipMemcpy(I.exFaces, faces.data(), faces.size() * sizeof(uint2), hipMemcpyHostToDevice), "upload extra faces");
   if (!cells.empty()) HIPCHK(hipMemcpy(I.fcCells, cells.data(), cells.size() * sizeof(int2), hipMemcpyHostToDevice), "upload far cells");
   HIPCHK(hipMemcpy(I.exStart, start.data(), start.size() * sizeof(int), hipMemcpyHostToDevice), "upload extra starts");
@@ -325,19 +396,19 @@ int inst_set_shape(Inst& I, int shape) {
   I.tilesX = (I.Nx + TX - 1) / TX;
   I.tilesY = (I.Nr + TY - 1) / TY;
   const size_t tiles = (size_t)I.tilesX * I.tilesY;
-  hfree(I.exStart);
-  hfree(I.fcStart);
-  HIPCHK(hipMalloc(&I.exStart, (tiles + 1) * sizeof(int)), "hipMalloc extra starts");
-  HIPCHK(hipMalloc(&I.fcStart, (tiles + 1) * sizeof(int)), "hipMalloc far starts");
+  dfree(I, I.exStart);
+  dfree(I, I.fcStart);
+  HIPCHK(dalloc(I, &I.exStart, (tiles + 1) * sizeof(int)), "allocate extra starts");
+  HIPCHK(dalloc(I, &I.fcStart, (tiles + 1) * sizeof(int)), "allocate far starts");
   if (int rc = build_extra_faces(I, I.codeH)) return rc;
   HIPCHK(hipMemcpy(I.code, I.codeH.data(), I.codeH.size() * 4, hipMemcpyHostToDevice), "upload code");
   return 0;
 }
 
 int inst_spray_alloc(Inst& I, int NP, hipStream_t s) {
-  hfree(I.parc);
+  dfree(I, I.parc);
   I.NP = NP;
-  HIPCHK(hipMalloc(&I.parc, (size_t)NP * 8 * 4), "hipMalloc parcels");
+  HIPCHK(dalloc(I, &I.parc, (size_t)NP * 8 * 4), "allocate parcels");
   I.bytes = inst_bytes(I.Nx, I.Nr, I.prec, NP);
   return inst_spray_reset(I, s);
 }
@@ -386,7 +457,7 @@ int inst_cfl_init(Inst& I, hipStream_t s) {
 int inst_set_state(Inst& I, const float* A, const float* B, hipStream_t s) {
   const long long N = (long long)I.Nx * I.Nr;
   float* d = nullptr;
-  HIPCHK(hipMalloc(&d, (size_t)N * 32), "hipMalloc upload");
+  HIPCHK(dalloc(I, &d, (size_t)N * 32), "allocate upload");
   int rc = 0;
   do {
     if (hipMemcpy(d, A, (size_t)N * 16, hipMemcpyHostToDevice) != hipSuccess ||
@@ -401,7 +472,7 @@ int inst_set_state(Inst& I, const float* A, const float* B, hipStream_t s) {
       break;
     }
   } while (false);
-  (void)hipFree(d);
+  dfree(I, d);  // the stream is done with it (synchronised above, or a copy failed first)
   if (rc) return rc;
   // zero step info and time; the spray is reset, its seed carries on (spray.js:242-254)
   InstState is{};
@@ -452,7 +523,7 @@ void fill_consts(const Inst& I, const nz_params& P, Consts& c, double tEndGrid, 
   c.maxSteps = maxSteps;
 }
 
-int inst_integrals(const Inst& I, const DevInst& d, const nz_probes& pr, hipStream_t s, nz_integrals* out) {
+nzk::IntegCtx integ_ctx(const DevInst& d, const nz_probes& pr) {
   nzk::IntegCtx q{};
   q.iExit = pr.iExit; q.jExit = pr.jExit; q.iThroat = pr.iThroat; q.iChamber = pr.iChamber;
   q.inj = d.c.inj;
@@ -465,6 +536,11 @@ int inst_integrals(const Inst& I, const DevInst& d, const nz_probes& pr, hipStre
     for (int k = 0; k < 3; ++k) q.Rs[k] = d.c.Rs[k];
     q.RaN = d.c.gas[0];
   }
+  return q;
+}
+
+int inst_integrals(const Inst& I, const DevInst& d, const nz_probes& pr, hipStream_t s, nz_integrals* out) {
+  const nzk::IntegCtx q = integ_ctx(d, pr);
   HIPCHK(nzk::launch_integrals(I.prec == 64, d, q, I.dint + 16, I.dint, s), "integrals");
   double v[16];
   HIPCHK(hipMemcpyAsync(v, I.dint, 13 * 8, hipMemcpyDeviceToHost, s), "integrals copy");

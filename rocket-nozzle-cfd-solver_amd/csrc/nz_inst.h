@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -18,6 +19,30 @@
   } while (0)
 
 namespace nz {
+
+// One device allocation carved into an instance's buffers: first fit on 256-byte
+// boundaries, freed blocks coalesced with their neighbours. Allocation and release are
+// host bookkeeping only (no runtime call, so no implicit device synchronisation); the
+// caller releases a block only once no queued work can still touch it.
+class DevArena {
+ public:
+  int create(size_t bytes);
+  void destroy();
+  void* alloc(size_t bytes);   // nullptr when no free block is large enough
+  void release(void* p);
+  size_t used() const { return used_; }
+  size_t size() const { return size_; }
+  // whether an allocation failed since clear_failed (a caller's "no room now")
+  bool failed() const { return failed_; }
+  void clear_failed() { failed_ = false; }
+
+ private:
+  char* base_ = nullptr;
+  size_t size_ = 0, used_ = 0;
+  bool failed_ = false;
+  std::map<size_t, size_t> free_;  // offset -> bytes
+  std::map<size_t, size_t> live_;  // offset -> bytes
+};
 
 // code word of a cell whose host is itself (offset fields hold dx+1 = dy+1 = 1)
 constexpr uint32_t kZeroOffset = (1u << nzk::C_HX_SHIFT) | (1u << nzk::C_HY_SHIFT);
@@ -56,11 +81,16 @@ struct Inst {
   int exMax = 0, fcMax = 0;
   nzk::Consts c{};
   size_t bytes = 0;  // device bytes held
+  DevArena* arena = nullptr;  // buffers carved from it (batched solves); else hipMalloc / hipFree
 };
 
 // device bytes an instance of this size needs (admission by HBM budget)
 size_t inst_bytes(int Nx, int Nr, int prec, int NP);
-int inst_alloc(Inst& I, int Nx, int Nr, int prec, nzk::InstState* st, int shape = 0);
+// arena: NULL for buffers of their own, else every buffer comes from it
+int inst_alloc(Inst& I, int Nx, int Nr, int prec, nzk::InstState* st, int shape = 0, DevArena* arena = nullptr);
+// device memory of an instance: from its arena, else the runtime's allocator
+hipError_t inst_dalloc(Inst& I, void** p, size_t bytes);
+void inst_dfree(Inst& I, void* p);
 // switch the tile shape: the per-tile lists are rebuilt for it
 int inst_set_shape(Inst& I, int shape);
 void inst_free(Inst& I);
@@ -79,6 +109,8 @@ void fill_consts(const Inst& I, const nz_params& P, nzk::Consts& c, double tEndG
                  int maxSteps);
 // descriptor with tile/parcel/cell block offsets 0 (single-instance launches)
 nzk::DevInst inst_desc(const Inst& I);
+// the integrals kernels' probe positions and gas data (the probes' own when exact)
+nzk::IntegCtx integ_ctx(const nzk::DevInst& d, const nz_probes& pr);
 // exitPerformance / chamberPerformance sums on the device (FP64), scaled to SI;
 // `d` carries the constants in use (ambient pressure); synchronises `s`
 int inst_integrals(const Inst& I, const nzk::DevInst& d, const nz_probes& pr, hipStream_t s, nz_integrals* out);

@@ -189,6 +189,7 @@ SIGNATURES = {
     "nz_batch_instance_bytes": (C.c_uint64, [C.c_int, C.c_int, C.c_int, C.c_int]),
     "nz_batch_submit": (C.c_int, [vp, P(NzBatchJob), P(C.c_int64)]),
     "nz_batch_run": (C.c_int, [vp, C.c_int, P(C.c_int), P(C.c_int)]),
+    "nz_batch_gpu_busy": (C.c_int, [vp, P(C.c_double)]),
     "nz_batch_result_get": (C.c_int, [vp, C.c_int64, P(NzBatchResult)]),
     "nz_batch_read_state": (C.c_int, [vp, C.c_int64, fp, fp]),
     "nz_batch_release": (C.c_int, [vp, C.c_int64]),

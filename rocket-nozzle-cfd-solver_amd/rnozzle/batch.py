@@ -20,7 +20,8 @@ def _f(a):
 
 
 class Batch:
-    def __init__(self, precision=32, budget_bytes=64 << 30, max_resident=256, graph=True):
+    def __init__(self, precision=32, budget_bytes=16 << 30, max_resident=256, graph=True):
+        """`budget_bytes`: device memory for all residents, allocated at once as one slab."""
         h = C.c_void_p()
         check(lib().nz_batch_create(C.byref(h), precision, int(budget_bytes), max_resident), "nz_batch_create")
         self._h = h
@@ -60,10 +61,18 @@ class Batch:
         return jid.value
 
     def run(self, steps):
-        """Admit, advance every resident `steps` steps, retire; returns (resident, queued)."""
+        """Admit, enqueue `steps` steps of every resident, then read the previous run's
+        stop flags and retire what stopped (the device runs this call's steps meanwhile);
+        returns (solves not yet filed, queued jobs). A loop of calls ends at (0, 0)."""
         r, q = C.c_int(), C.c_int()
         check(lib().nz_batch_run(self._h, steps, C.byref(r), C.byref(q)), "nz_batch_run")
         return r.value, q.value
+
+    def gpu_busy(self):
+        """Device seconds of the run chunks read so far (event-timed)."""
+        v = C.c_double()
+        check(lib().nz_batch_gpu_busy(self._h, C.byref(v)), "nz_batch_gpu_busy")
+        return v.value
 
     def finished(self):
         n = lib().nz_batch_finished(self._h, None, 0)
