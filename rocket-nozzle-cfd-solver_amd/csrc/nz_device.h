@@ -430,7 +430,7 @@ __device__ __forceinline__ void face_flux(Prim<R> c0, Prim<R> c1, Prim<R> c2, Pr
   const R mK = rK * uK;
   const R iSKs = rc(SK - Ss);
   const R kk = rK * (SK - uK) * iSKs;
-  const R irK = rc(rK);
+  const R irK = left ? irL : irR;  // 1 / rK, already formed for the side it takes
   const R Us0 = kk, Us1 = kk * Ss, Us2 = kk * vK, Us3 = kk * (EK * irK + (Ss - uK) * (Ss + dv(pK * irK, SK - uK)));
   F[0] = FK0 + SK * (Us0 - rK);
   F[1] = FK1 + SK * (Us1 - mK);
