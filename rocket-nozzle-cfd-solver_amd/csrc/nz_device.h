@@ -347,6 +347,21 @@ __device__ __forceinline__ R van_leer(R a, R b) {
   if (fabs(den) < R(1e-30)) den = R(1);
   return dv(R(2) * fmax(a * b, R(0)), den);
 }
+// MUSCL face value c +- half the van Leer slope of the differences a, b (solver.js:198-203):
+// c + s 0.5 (2 max(ab, 0) / (a + b)) = c + s max(ab, 0) / (a + b). FP32 folds the two
+// factors of 2 away and forms it as one multiply-add on the hardware reciprocal, whose
+// magnitude is capped at 1e30 in place of the reference's "a + b below 1e-30 -> 1" (where
+// max(ab, 0) < 2.5e-61, both add nothing to c); FP64 keeps the reference's form.
+template <bool PLUS>
+__device__ __forceinline__ float muscl(float c, float a, float b) {
+  const float m = fmaxf(a * b, 0.f);
+  const float r = __builtin_amdgcn_fmed3f(__builtin_amdgcn_rcpf(a + b), -1e30f, 1e30f);
+  return fmaf(PLUS ? m : -m, r, c);
+}
+template <bool PLUS>
+__device__ __forceinline__ double muscl(double c, double a, double b) {
+  return PLUS ? c + 0.5 * van_leer(a, b) : c - 0.5 * van_leer(a, b);
+}
 template <typename R>
 __device__ __forceinline__ R jump_ratio(R a, R b) { return dv(fabs(a - b), fmin(a, b)); }
 
@@ -368,25 +383,25 @@ __device__ __forceinline__ void face_flux(Prim<R> c0, Prim<R> c1, Prim<R> c2, Pr
     WL = c1;
     WR = c2;
     if (second) {
-      const R tLr = c1.r + R(0.5) * van_leer(c1.r - c0.r, c2.r - c1.r);
-      const R tLu = c1.u + R(0.5) * van_leer(c1.u - c0.u, c2.u - c1.u);
-      const R tLv = c1.v + R(0.5) * van_leer(c1.v - c0.v, c2.v - c1.v);
-      const R tLp = c1.p + R(0.5) * van_leer(c1.p - c0.p, c2.p - c1.p);
-      const R tRr = c2.r - R(0.5) * van_leer(c2.r - c1.r, c3.r - c2.r);
-      const R tRu = c2.u - R(0.5) * van_leer(c2.u - c1.u, c3.u - c2.u);
-      const R tRv = c2.v - R(0.5) * van_leer(c2.v - c1.v, c3.v - c2.v);
-      const R tRp = c2.p - R(0.5) * van_leer(c2.p - c1.p, c3.p - c2.p);
+      const R tLr = muscl<true>(c1.r, c1.r - c0.r, c2.r - c1.r);
+      const R tLu = muscl<true>(c1.u, c1.u - c0.u, c2.u - c1.u);
+      const R tLv = muscl<true>(c1.v, c1.v - c0.v, c2.v - c1.v);
+      const R tLp = muscl<true>(c1.p, c1.p - c0.p, c2.p - c1.p);
+      const R tRr = muscl<false>(c2.r, c2.r - c1.r, c3.r - c2.r);
+      const R tRu = muscl<false>(c2.u, c2.u - c1.u, c3.u - c2.u);
+      const R tRv = muscl<false>(c2.v, c2.v - c1.v, c3.v - c2.v);
+      const R tRp = muscl<false>(c2.p, c2.p - c1.p, c3.p - c2.p);
       if (tLr > R(0) && tLp > R(0) && tRr > R(0) && tRp > R(0)) {
         WL.r = tLr; WL.u = tLu; WL.v = tLv; WL.p = tLp;
         WR.r = tRr; WR.u = tRu; WR.v = tRv; WR.p = tRp;
-        WL.Z = clampR(c1.Z + R(0.5) * van_leer(c1.Z - c0.Z, c2.Z - c1.Z), R(0), R(1));
-        WL.F = clampR(c1.F + R(0.5) * van_leer(c1.F - c0.F, c2.F - c1.F), R(0), R(1));
-        WL.G = clampR(c1.G + R(0.5) * van_leer(c1.G - c0.G, c2.G - c1.G), R(0.5), R(100));
-        WL.O = clampR(c1.O + R(0.5) * van_leer(c1.O - c0.O, c2.O - c1.O), R(0), R(1));
-        WR.Z = clampR(c2.Z - R(0.5) * van_leer(c2.Z - c1.Z, c3.Z - c2.Z), R(0), R(1));
-        WR.F = clampR(c2.F - R(0.5) * van_leer(c2.F - c1.F, c3.F - c2.F), R(0), R(1));
-        WR.G = clampR(c2.G - R(0.5) * van_leer(c2.G - c1.G, c3.G - c2.G), R(0.5), R(100));
-        WR.O = clampR(c2.O - R(0.5) * van_leer(c2.O - c1.O, c3.O - c2.O), R(0), R(1));
+        WL.Z = clampR(muscl<true>(c1.Z, c1.Z - c0.Z, c2.Z - c1.Z), R(0), R(1));
+        WL.F = clampR(muscl<true>(c1.F, c1.F - c0.F, c2.F - c1.F), R(0), R(1));
+        WL.G = clampR(muscl<true>(c1.G, c1.G - c0.G, c2.G - c1.G), R(0.5), R(100));
+        WL.O = clampR(muscl<true>(c1.O, c1.O - c0.O, c2.O - c1.O), R(0), R(1));
+        WR.Z = clampR(muscl<false>(c2.Z, c2.Z - c1.Z, c3.Z - c2.Z), R(0), R(1));
+        WR.F = clampR(muscl<false>(c2.F, c2.F - c1.F, c3.F - c2.F), R(0), R(1));
+        WR.G = clampR(muscl<false>(c2.G, c2.G - c1.G, c3.G - c2.G), R(0.5), R(100));
+        WR.O = clampR(muscl<false>(c2.O, c2.O - c1.O, c3.O - c2.O), R(0), R(1));
       }
     }
   }
