@@ -364,6 +364,17 @@ struct StageSmem {
   static constexpr int PW = TX + 4, PH = TY + 4;   // primitive tile with 2-cell halo
   static constexpr int NFX = (TX + 1) * TY, NFR = TX * (TY + 1);
   static constexpr int TW = TX + 2, THh = TY + 2;  // T1/T2 tile with 1-cell ring
+  // the halo's cells outside the tile: rows 0, 1, then rows PH - 2, PH - 1 (each left to
+  // right), then columns 0, 1, PW - 2, PW - 1 of rows 2 .. PH - 3, row by row
+  static constexpr int NRING = PW * PH - TX * TY;
+  static __device__ __forceinline__ int2 ring_cell(int q) {
+    if (q < 2 * PW) return make_int2(q % PW, q / PW);
+    q -= 2 * PW;
+    if (q < 2 * PW) return make_int2(q % PW, PH - 2 + q / PW);
+    q -= 2 * PW;
+    const int c = q & 3;
+    return make_int2(c < 2 ? c : PW - 4 + c, 2 + (q >> 2));
+  }
   R P[8][PH][PW];
   float AX[TY][TX + 1];                            // signed areas of the tile's x faces
   float AR[TY + 1][TX];                            // ... and r faces (0 outside the domain)
@@ -476,8 +487,9 @@ __global__ void __launch_bounds__(TX* TY, (STAGE == 2 && PHYS) ? 1 : W)
   // loads depend only on the tile; on the latency-tuned shapes they are issued ahead of
   // the step state (which decides whether the solve is still running and gives the time
   // the ghost rules need), so that its round trip overlaps them instead of preceding them.
-  // This thread's own cell: its code word, group geometry and state, for the update
-  // phase; loaded here so that their latency overlaps the tile load and the face loop
+  // This thread's own cell: its group geometry, for the update phase, loaded here so that
+  // its latency overlaps the tile load and the face loop (its code word and state come with
+  // the tile's cells below: a thread's first tile cell is its own)
   const int tx = tid % TX, ty = tid / TX;
   const int ci = i0 + tx, cj = j0 + ty;
   const bool inDom = ci < Nx && cj < Nr;
@@ -485,15 +497,11 @@ __global__ void __launch_bounds__(TX* TY, (STAGE == 2 && PHYS) ? 1 : W)
   // Every load below is issued by every lane, from an address clamped into its array, and
   // lanes with nothing to load discard the value: loads under a divergent branch would be
   // waited for inside it, one round trip per branch instead of one for the whole set.
-  uint32_t code = C_IS_SLAVE;
   float4 gg = make_float4(0.f, 0.f, 0.f, 0.f);
-  R a[4], b[4];
   {
     const unsigned kc = (unsigned)(min(cj, Nr - 1) * Nx + min(ci, Nx - 1));
-    const uint32_t cd = gld(I.code, kc);
     const float4 g4 = gld(I.gg, kc);
-    for (int q = 0; q < 4; ++q) { a[q] = gld(Uin + q * Ns, kc); b[q] = gld(Uin + (4 + q) * Ns, kc); }
-    if (inDom) { code = cd; gg = g4; }
+    if (inDom) gg = g4;
   }
   // eddy-viscosity length scales of this thread's cells of the viscous ring tile
   constexpr int LPT = (S::TW * S::THh + NT - 1) / NT;
@@ -513,14 +521,16 @@ __global__ void __launch_bounds__(TX* TY, (STAGE == 2 && PHYS) ? 1 : W)
   const int nEx = min(cld(I.exStart, (unsigned)lt_ + 1u) - exBase, S::MAXE);
   const int fcBase = cld(I.fcStart, (unsigned)lt_);
   const int nF = min(cld(I.fcStart, (unsigned)lt_ + 1u) - fcBase, S::MAXF);
-  // every load of this thread's cells and face areas is issued before any is used
-  constexpr int CPT = (S::PW * S::PH + NT - 1) / NT, FPT = (S::NFX + S::NFR + NT - 1) / NT;
+  // every load of this thread's cells and face areas is issued before any is used: its own
+  // cell (tile position (tx, ty) + 2), then ring cells of the 2-cell halo (tile_ring_cell)
+  constexpr int CPT = 1 + (S::NRING + NT - 1) / NT, FPT = (S::NFX + S::NFR + NT - 1) / NT;
   CellRaw<R> raw[CPT];
   float area[FPT];
+  raw[0] = cell_src_load<R>(I, Uin, ci, cj);
 #pragma unroll
-  for (int r = 0; r < CPT; ++r) {
-    const int idx = min(tid + r * NT, S::PW * S::PH - 1);  // lanes past the halo reload its last cell
-    raw[r] = cell_src_load<R>(I, Uin, i0 - 2 + idx % S::PW, j0 - 2 + idx / S::PW);
+  for (int r = 1; r < CPT; ++r) {
+    const int2 lc = S::ring_cell(min(tid + (r - 1) * NT, S::NRING - 1));  // lanes past the ring reload its last cell
+    raw[r] = cell_src_load<R>(I, Uin, i0 - 2 + lc.x, j0 - 2 + lc.y);
   }
 #pragma unroll
   for (int r = 0; r < FPT; ++r) {
@@ -542,8 +552,8 @@ __global__ void __launch_bounds__(TX* TY, (STAGE == 2 && PHYS) ? 1 : W)
     const float ld = gld(base, (unsigned)(min(fj, dirR ? Nr : Nr - 1) * row + min(fi, dirR ? Nx - 1 : Nx)));
     area[r] = (in && tid + r * NT < S::NFX + S::NFR) ? ld : 0.f;
   }
-  // the last step's partials (stage 1): loaded with the tile, folded once the tile's
-  // primitives are in LDS, so that their round trip overlaps the tile's
+  // the last step's partials (stage 1): loaded with the tile, so that their round trip
+  // overlaps the tile's, and folded after the face fluxes
   // (tiles of fewer than 256 threads hold more partials per thread: they load them late)
   constexpr bool EARLY_FOLD = NT >= 256;
   FoldLoads<NT> fl;
@@ -561,9 +571,10 @@ __global__ void __launch_bounds__(TX* TY, (STAGE == 2 && PHYS) ? 1 : W)
   if (active) {
 #pragma unroll
     for (int r = 0; r < CPT; ++r) {
-      const int idx = tid + r * NT;
-      if (idx < S::PW * S::PH) {
-        const int li = idx % S::PW, lj = idx / S::PW;
+      const int idx = tid + (r - 1) * NT;
+      if (r == 0 || idx < S::NRING) {
+        const int2 lc = r == 0 ? make_int2(tx + 2, ty + 2) : S::ring_cell(idx);
+        const int li = lc.x, lj = lc.y;
         const Prim<R> q = prim_from<R>(I, raw[r], i0 - 2 + li, j0 - 2 + lj, tsel);
         sm.P[0][lj][li] = q.r; sm.P[1][lj][li] = q.u; sm.P[2][lj][li] = q.v; sm.P[3][lj][li] = q.p;
         sm.P[4][lj][li] = q.Z; sm.P[5][lj][li] = q.F; sm.P[6][lj][li] = q.G; sm.P[7][lj][li] = q.O;
@@ -587,10 +598,6 @@ __global__ void __launch_bounds__(TX* TY, (STAGE == 2 && PHYS) ? 1 : W)
       const Prim<R> q = prim_at<R>(I, Uin, fcell.x, fcell.y, tsel);
       sm.PF[0][e] = q.r; sm.PF[1][e] = q.u; sm.PF[2][e] = q.v; sm.PF[3][e] = q.p;
       sm.PF[4][e] = q.Z; sm.PF[5][e] = q.F; sm.PF[6][e] = q.G; sm.PF[7][e] = q.O;
-    }
-    if (STAGE == 1 && fused) {
-      if constexpr (EARLY_FOLD) fold_finish<NT>(I.part, nPart, tid, fl, sm.foldW, sm.foldR);
-      else fold_groups<NT>(I.part, nPart, tid, sm.foldW, sm.foldR);
     }
     if (nF > 0) {  // uniform over the workgroup
       for (int w = tid; w < S::WW * S::WH; w += NT) sm.fmap[w / S::WW][w % S::WW] = 0xFF;
@@ -630,28 +637,6 @@ __global__ void __launch_bounds__(TX* TY, (STAGE == 2 && PHYS) ? 1 : W)
       return prim_at<R>(I, Uin, gi, gj, tsel);
     };
     __syncthreads();
-
-    // this step's time step (stage 1); the first workgroup records it, with the folded
-    // wave speed and residual of the last step and that step's residual stop test
-    if (STAGE == 1) {
-      double wF = 0.0, rF = 0.0;
-      if (fused) fold_combine(sm.foldW, sm.foldR, wF, rF);
-      const double wmax = fused ? wF : __longlong_as_double((long long)sv.wsnap);
-      step_info<R>(c, sv.last0, sv.last2, wmax, cur);
-      dt = cur[0];
-      if (lt_ == 0 && tid == 0) {
-        st->cur[0] = cur[0]; st->cur[1] = cur[1]; st->cur[2] = cur[2]; st->cur[3] = cur[3];
-        // the parcels this step spawns weigh sprayMhat dt / 4: the bound of the deposit's
-        // fixed-point scale, which this step's spray and stage 2 read
-        if (c.spray) st->mmax = fmax(st->mmax, (double)(float)(R(c.sprayMhat) * cur[0] / R(4)));
-        if (fused) {
-          st->wsnap = (unsigned long long)__double_as_longlong(wF);
-          st->resAcc = rF;
-          st->resLast = sqrt(rF / fmax(st->nGroups, 1.0)) * (double)c.resScale;
-          st->flags |= stop_bits(c, *st, st->resLast);
-        }
-      }
-    }
 
     // ---- 2. each face flux once (FS_FLUX_X / FS_FLUX_R), times |area|: the west and
     // the south face of every tile cell (one each per thread, direction fixed at compile
@@ -705,7 +690,36 @@ __global__ void __launch_bounds__(TX* TY, (STAGE == 2 && PHYS) ? 1 : W)
         gst(exFlux, e * 16 + 8, FO);
       }
     }
+    // the last step's partials (stage 1), folded once this wave's faces are formed: the fold
+    // is a chain of cross-lane steps with little arithmetic, whose latency the other waves'
+    // face work now hides (the time step it gives is first needed by the update below)
+    if (STAGE == 1 && fused) {
+      if constexpr (EARLY_FOLD) fold_finish<NT>(I.part, nPart, tid, fl, sm.foldW, sm.foldR);
+      else fold_groups<NT>(I.part, nPart, tid, sm.foldW, sm.foldR);
+    }
     __syncthreads();
+
+    // this step's time step (stage 1); the first workgroup records it, with the folded
+    // wave speed and residual of the last step and that step's residual stop test
+    if (STAGE == 1) {
+      double wF = 0.0, rF = 0.0;
+      if (fused) fold_combine(sm.foldW, sm.foldR, wF, rF);
+      const double wmax = fused ? wF : __longlong_as_double((long long)sv.wsnap);
+      step_info<R>(c, sv.last0, sv.last2, wmax, cur);
+      dt = cur[0];
+      if (lt_ == 0 && tid == 0) {
+        st->cur[0] = cur[0]; st->cur[1] = cur[1]; st->cur[2] = cur[2]; st->cur[3] = cur[3];
+        // the parcels this step spawns weigh sprayMhat dt / 4: the bound of the deposit's
+        // fixed-point scale, which this step's spray and stage 2 read
+        if (c.spray) st->mmax = fmax(st->mmax, (double)(float)(R(c.sprayMhat) * cur[0] / R(4)));
+        if (fused) {
+          st->wsnap = (unsigned long long)__double_as_longlong(wF);
+          st->resAcc = rF;
+          st->resLast = sqrt(rF / fmax(st->nGroups, 1.0)) * (double)c.resScale;
+          st->flags |= stop_bits(c, *st, st->resLast);
+        }
+      }
+    }
 
     // weighted inviscid flux of face f (W, E, S, N = 0..3) of member cell (mi, mj):
     // the in-tile array for faces of the tile, the slave's extra-face slots (`slot`,
@@ -726,7 +740,11 @@ __global__ void __launch_bounds__(TX* TY, (STAGE == 2 && PHYS) ? 1 : W)
       }
     };
 
-    // ---- 3. merged-group finite-volume update (FS_UPDATE)
+    // ---- 3. merged-group finite-volume update (FS_UPDATE); the thread's own cell's
+    // state and code word came with the tile (raw[0])
+    const uint32_t code = inDom ? raw[0].code : (uint32_t)C_IS_SLAVE;
+    R a[4], b[4];
+    for (int q = 0; q < 4; ++q) { a[q] = raw[0].v[q]; b[q] = raw[0].v[4 + q]; }
     const bool solid = (code & C_SOLID) != 0;
     const bool host = inDom && !solid && !(code & C_IS_SLAVE);
     R A[4], B[4];         // group state after the stage
