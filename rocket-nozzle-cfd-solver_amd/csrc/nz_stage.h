@@ -748,40 +748,48 @@ __global__ void __launch_bounds__(TX* TY, (STAGE == 2 && PHYS) ? 1 : W)
     const bool solid = (code & C_SOLID) != 0;
     const bool host = inDom && !solid && !(code & C_IS_SLAVE);
     R A[4], B[4];         // group state after the stage
+    // Group members in fixed slots: 0 the host, 1..4 its west, east, south and north
+    // neighbours, present as its slave bits say (cutcell.js:168-174). The short loops over
+    // them are unrolled over the five slots and the face loops take a member's cell from its
+    // slot number, so no array here is indexed at run time (which would put it in scratch).
+    uint32_t mem = 0u;    // bit m: member slot m present
     R ubarM[5];           // member-specific B.w (inflow-speed average in column 0)
-    int mi[5], mj[5], nm = 0;
-    int mslot[5];         // first extra-face slot of each member (-1: faces in the tile or none)
+    int mi[5], mj[5];
+    mi[0] = ci; mj[0] = cj; mi[1] = ci - 1; mj[1] = cj; mi[2] = ci + 1; mj[2] = cj;
+    mi[3] = ci; mj[3] = cj - 1; mi[4] = ci; mj[4] = cj + 1;
+    auto mem_i = [&](int m) { return ci + (m == 1 ? -1 : m == 2 ? 1 : 0); };
+    auto mem_j = [&](int m) { return cj + (m == 3 ? -1 : m == 4 ? 1 : 0); };
+    // slaves outside the tile take 3 consecutive extra-face slots each, in member order: the
+    // first slot of member m, with es the running slot (from the host's code word)
+    auto mem_slot = [&](int m, int& es) {
+      const int xi = mem_i(m), yj = mem_j(m);
+      const bool inT = xi >= i0 && xi < i0 + TX && yj >= j0 && yj < j0 + TY;
+      const int slot = (inT || es < 0) ? -1 : es;
+      if (!inT && es >= 0) es += 3;
+      return slot;
+    };
     bool guard = false;   // NaN guard hit
     R rho_old = R(0);
     Prim<R> qS;           // state after the stage, as primitives (physics input)
     if (host) {
-      mi[0] = ci; mj[0] = cj; mslot[0] = -1; nm = 1;
-      const uint32_t sl = code_slaves(code);
-      if (sl & 1u) { mi[nm] = ci - 1; mj[nm] = cj; ++nm; }
-      if (sl & 2u) { mi[nm] = ci + 1; mj[nm] = cj; ++nm; }
-      if (sl & 4u) { mi[nm] = ci; mj[nm] = cj - 1; ++nm; }
-      if (sl & 8u) { mi[nm] = ci; mj[nm] = cj + 1; ++nm; }
-      // slaves outside the tile take 3 consecutive extra-face slots each, in member order
-      int es = code_exslot(code) * 3;
-      for (int m = 1; m < nm; ++m) {
-        const bool inT = mi[m] >= i0 && mi[m] < i0 + TX && mj[m] >= j0 && mj[m] < j0 + TY;
-        mslot[m] = (inT || es < 0) ? -1 : es;
-        if (!inT && es >= 0) es += 3;
-      }
+      mem = 1u | (code_slaves(code) << 1);
       R L[4], LY[4], LO;
       for (int q = 0; q < 4; ++q) {
         L[q] = -(sm.u.inv.Fx[q][ty][tx + 1] - sm.u.inv.Fx[q][ty][tx]) - (sm.u.inv.Fr[q][ty + 1][tx] - sm.u.inv.Fr[q][ty][tx]);
         LY[q] = -(sm.u.inv.Fx[4 + q][ty][tx + 1] - sm.u.inv.Fx[4 + q][ty][tx]) - (sm.u.inv.Fr[4 + q][ty + 1][tx] - sm.u.inv.Fr[4 + q][ty][tx]);
       }
       LO = -(sm.u.inv.Fx[8][ty][tx + 1] - sm.u.inv.Fx[8][ty][tx]) - (sm.u.inv.Fr[8][ty + 1][tx] - sm.u.inv.Fr[8][ty][tx]);
-      for (int m = 1; m < nm; ++m) {
+      int es = code_exslot(code) * 3;
+      for (int m = 1; m < 5; ++m) {
+        if (!((mem >> m) & 1u)) continue;
         // the slave's own net flux: west/south faces enter, east/north faces leave
         R Fw[9], Fe_[9], Fs[9], Fn[9];
-        int slot = mslot[m];
-        invFace(0, mi[m], mj[m], slot, Fw);
-        invFace(1, mi[m], mj[m], slot, Fe_);
-        invFace(2, mi[m], mj[m], slot, Fs);
-        invFace(3, mi[m], mj[m], slot, Fn);
+        int slot = mem_slot(m, es);
+        const int xi = mem_i(m), yj = mem_j(m);
+        invFace(0, xi, yj, slot, Fw);
+        invFace(1, xi, yj, slot, Fe_);
+        invFace(2, xi, yj, slot, Fs);
+        invFace(3, xi, yj, slot, Fn);
         for (int q = 0; q < 4; ++q) {
           L[q] += -(Fe_[q] - Fw[q]) - (Fn[q] - Fs[q]);
           LY[q] += -(Fe_[4 + q] - Fw[4 + q]) - (Fn[4 + q] - Fs[4 + q]);
@@ -820,7 +828,7 @@ __global__ void __launch_bounds__(TX* TY, (STAGE == 2 && PHYS) ? 1 : W)
       if (!(rho < R(1e30)) || !(vx < R(1e30) && vx > R(-1e30))) {
         guard = true;
         for (int m = 0; m < 4; ++m) { A[m] = R(c.ambCons[m]); B[m] = R(c.ambB[m]); }
-        for (int m = 0; m < nm; ++m) ubarM[m] = R(c.ambB[3]);
+        for (int m = 0; m < 5; ++m) ubarM[m] = R(c.ambB[3]);
       } else {
         const R ke = R(0.5) * rho * (vx * vx + vy * vy);
         const R Gn = clampR(UY[2], R(c.gamR[0]), R(c.gamR[1]));
@@ -829,7 +837,9 @@ __global__ void __launch_bounds__(TX* TY, (STAGE == 2 && PHYS) ? 1 : W)
         A[0] = rho; A[1] = rho * vx; A[2] = rho * vy; A[3] = E;
         B[0] = Zn; B[1] = clampR(UY[1], R(0), Zn); B[2] = Gn;
         B[3] = c.inj ? clampR(UY[3], R(0), rho) : R(0);
-        for (int m = 0; m < nm; ++m) {
+#pragma unroll
+        for (int m = 0; m < 5; ++m) {
+          if (!((mem >> m) & 1u)) continue;  // an absent slot's cell may lie outside the grid
           R ub = B[3];
           if (!c.inj) {
             ub = R(0);
@@ -977,13 +987,16 @@ __global__ void __launch_bounds__(TX* TY, (STAGE == 2 && PHYS) ? 1 : W)
         R T1h[4], T2h[4];
         for (int q = 0; q < 4; ++q) { T1h[q] = sm.u.vis.T[q][ty + 1][tx + 1]; T2h[q] = sm.u.vis.T[4 + q][ty + 1][tx + 1]; }
         if (c.visc) {
-          for (int m = 0; m < nm; ++m) {
+          int es = code_exslot(code) * 3;
+          for (int m = 0; m < 5; ++m) {
+            if (!((mem >> m) & 1u)) continue;
             R Fw[7], Fe_[7], Fs[7], Fn[7];
-            int slot = mslot[m];
-            visFace(0, mi[m], mj[m], slot, Fw);
-            visFace(1, mi[m], mj[m], slot, Fe_);
-            visFace(2, mi[m], mj[m], slot, Fs);
-            visFace(3, mi[m], mj[m], slot, Fn);
+            int slot = m == 0 ? -1 : mem_slot(m, es);
+            const int xi = mem_i(m), yj = mem_j(m);
+            visFace(0, xi, yj, slot, Fw);
+            visFace(1, xi, yj, slot, Fe_);
+            visFace(2, xi, yj, slot, Fs);
+            visFace(3, xi, yj, slot, Fn);
             for (int q = 0; q < 4; ++q) L[q] += -(Fe_[q] - Fw[q]) - (Fn[q] - Fs[q]);
             for (int q = 0; q < 3; ++q) LY[q] += -(Fe_[4 + q] - Fw[4 + q]) - (Fn[4 + q] - Fs[4 + q]);
           }
@@ -1055,7 +1068,7 @@ __global__ void __launch_bounds__(TX* TY, (STAGE == 2 && PHYS) ? 1 : W)
         if (!(rn < R(1e30)) || !(vnx < R(1e30) && vnx > R(-1e30))) {
           guard = true;
           for (int m = 0; m < 4; ++m) { A[m] = R(c.ambCons[m]); B[m] = R(c.ambB[m]); }
-          for (int m = 0; m < nm; ++m) ubarM[m] = R(c.ambB[3]);
+          for (int m = 0; m < 5; ++m) ubarM[m] = R(c.ambB[3]);
         } else {
           const R Gn = clampR(UY[2], R(c.gamR[0]), R(c.gamR[1]));
           const R Zn = clampR(UY[0], R(0), rn);
@@ -1064,7 +1077,7 @@ __global__ void __launch_bounds__(TX* TY, (STAGE == 2 && PHYS) ? 1 : W)
           B[0] = Zn; B[1] = clampR(UY[1], R(0), Zn); B[2] = Gn;
           if (c.inj) {
             B[3] = clampR(UY[3], R(0), rn);
-            for (int m = 0; m < nm; ++m) ubarM[m] = B[3];
+            for (int m = 0; m < 5; ++m) ubarM[m] = B[3];
           }
         }
       }
@@ -1100,7 +1113,7 @@ __global__ void __launch_bounds__(TX* TY, (STAGE == 2 && PHYS) ? 1 : W)
         B[2] = Gn;
         // every member takes the host's B (spray.js:161-177)
         const R bw = ubarM[0] + s1 * invV;
-        for (int m = 0; m < nm; ++m) ubarM[m] = bw;
+        for (int m = 0; m < 5; ++m) ubarM[m] = bw;
       }
     }
 
@@ -1110,7 +1123,9 @@ __global__ void __launch_bounds__(TX* TY, (STAGE == 2 && PHYS) ? 1 : W)
       for (int q = 0; q < 4; ++q) { gst(Uout + q * Ns, (unsigned)k, R(c.ambCons[q])); gst(Uout + (4 + q) * Ns, (unsigned)k, R(c.ambB[q])); }
     }
     if (host) {
-      for (int m = 0; m < nm; ++m) {
+#pragma unroll
+      for (int m = 0; m < 5; ++m) {
+        if (!((mem >> m) & 1u)) continue;
         const long long km = (long long)mj[m] * Nx + mi[m];
         for (int q = 0; q < 4; ++q) gst(Uout + q * Ns, (unsigned)km, A[q]);
         for (int q = 0; q < 3; ++q) gst(Uout + (4 + q) * Ns, (unsigned)km, B[q]);
