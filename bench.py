@@ -47,7 +47,7 @@ PARCEL_BYTES = 64 + 32 + 16   # parcel read+write, gas gather, 4-float deposit
 # HBM-side bytes per launch of the stage-2 kernel measured with rocprofv3 FETCH_SIZE and
 # WRITE_SIZE (scaled by a 256 MiB copy: x2.0 and x1.0), for (workload, grid, tile shape);
 # profiles/hot_bell_768x256_counters.md holds the runs
-MEASURED_TRAFFIC = {("hot_bell", "768x256", 1): 29.53e6}
+MEASURED_TRAFFIC = {("hot_bell", "768x256", 1): 27.86e6}
 
 
 def cpu_model():
