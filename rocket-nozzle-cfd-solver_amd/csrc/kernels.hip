@@ -452,16 +452,24 @@ __global__ void __launch_bounds__(256) k_integrals_cells(DevInst I, long long ru
     m += (double)U[k] * V;
     e += (double)U[3 * Ns + k] * V;
   }
-  const double ms = block_sum<256>(m, red);
+  // both sums folded as block_sum folds each (a butterfly per wave, then the waves in order)
+  for (int o = 32; o > 0; o >>= 1) {
+    m += __shfl_xor(m, o, 64);
+    e += __shfl_xor(e, o, 64);
+  }
+  if ((threadIdx.x & 63) == 0) { red[threadIdx.x >> 6] = m; red[4 + (threadIdx.x >> 6)] = e; }
   __syncthreads();
-  const double es = block_sum<256>(e, red);
-  if (threadIdx.x == 0) part[blockIdx.x] = make_double2(ms, es);
+  if (threadIdx.x == 0) {
+    double ms = 0.0, es = 0.0;
+    for (int k = 0; k < 4; ++k) { ms += red[k]; es += red[4 + k]; }
+    part[blockIdx.x] = make_double2(ms, es);
+  }
 }
 
 template <typename R>
 __global__ void __launch_bounds__(256) k_integrals_close(DevInst I, IntegCtx q, const double2* __restrict__ part, int nb,
                                                          double* __restrict__ out) {
-  __shared__ double red[8];
+  __shared__ double redn[10][4];
   __shared__ int jmin[4];
   const int tid = threadIdx.x;
   const R* __restrict__ U = static_cast<const R*>(I.U0);
@@ -502,12 +510,23 @@ __global__ void __launch_bounds__(256) k_integrals_close(DevInst I, IntegCtx q, 
     mO += (double)U[7 * Ns + k] * u * dA;
     mF += (double)U[5 * Ns + k] * u * dA;
   }
-  const double vals[13] = {F, md, Mw, pw, Aa, mt, mass, en, 0.0, 0.0, 0.0, mO, mF};
-  for (int n = 0; n < 13; ++n) {
-    if (n >= 8 && n <= 10) continue;
-    const double s = block_sum<256>(vals[n], red);
-    __syncthreads();
-    if (tid == 0) out[n] = s;
+  // the ten sums together: each is folded exactly as block_sum folds it (a butterfly per
+  // wave, then the four waves in order), with one barrier for all of them
+  constexpr int NS = 10;
+  double vals[NS] = {F, md, Mw, pw, Aa, mt, mass, en, mO, mF};
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1)
+#pragma unroll
+    for (int n = 0; n < NS; ++n) vals[n] += __shfl_xor(vals[n], o, 64);
+  if ((tid & 63) == 0)
+#pragma unroll
+    for (int n = 0; n < NS; ++n) redn[n][tid >> 6] = vals[n];
+  __syncthreads();
+  if (tid < NS) {
+    double m = 0.0;
+    for (int k = 0; k < 4; ++k) m += redn[tid][k];
+    constexpr int slot[NS] = {0, 1, 2, 3, 4, 5, 6, 7, 11, 12};
+    out[slot[tid]] = m;
   }
   if (tid == 0) {
     // chamber probe at (iChamber, 0)

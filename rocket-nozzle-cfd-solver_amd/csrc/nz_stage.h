@@ -48,9 +48,34 @@ __device__ __forceinline__ void step_info(const Consts& c, double last0, double 
   cur[3] = w;
 }
 
+// A double moved across the lanes of a wave by the DPP control CTRL (both 32-bit halves);
+// lanes of rows outside ROW_MASK take `id`, the identity of the reduction step.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ double dpp_move(double x, double id) {
+  const int lo = __builtin_amdgcn_update_dpp(__double2loint(id), __double2loint(x), CTRL, ROW_MASK, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(__double2hiint(id), __double2hiint(x), CTRL, ROW_MASK, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+// Max of w (>= 0) and sum of r over the 64 lanes of a wave, left in every lane, in one fixed
+// order: swaps within quads (lane ^ 1, lane ^ 2), the half-row and row mirrors (each row of
+// 16 lanes then holds its row's result in every lane), row 0 into row 1 and row 2 into
+// row 3 (row_bcast15), rows 0-1 into rows 2-3 (row_bcast31); lane 63 then holds the total,
+// which every lane reads. Register-to-register moves, where a butterfly of lane shuffles
+// is a chain of LDS round trips. Every lane of the wave must be active.
+__device__ __forceinline__ void wave_max_sum(double& w, double& r) {
+  w = fmax(w, dpp_move<0xB1, 0xF>(w, 0.0)); r += dpp_move<0xB1, 0xF>(r, 0.0);
+  w = fmax(w, dpp_move<0x4E, 0xF>(w, 0.0)); r += dpp_move<0x4E, 0xF>(r, 0.0);
+  w = fmax(w, dpp_move<0x141, 0xF>(w, 0.0)); r += dpp_move<0x141, 0xF>(r, 0.0);
+  w = fmax(w, dpp_move<0x140, 0xF>(w, 0.0)); r += dpp_move<0x140, 0xF>(r, 0.0);
+  w = fmax(w, dpp_move<0x142, 0xA>(w, 0.0)); r += dpp_move<0x142, 0xA>(r, 0.0);
+  w = fmax(w, dpp_move<0x143, 0xC>(w, 0.0)); r += dpp_move<0x143, 0xC>(r, 0.0);
+  w = __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(w), 63), __builtin_amdgcn_readlane(__double2loint(w), 63));
+  r = __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(r), 63), __builtin_amdgcn_readlane(__double2loint(r), 63));
+}
+
 // Wave-speed max and residual sum of an instance's tile partials, in one fixed order for
 // every caller: 256 virtual lanes v take partials v, v + 256, ... in turn; each group of
-// 64 lanes folds by a butterfly (which leaves the same bits in all its lanes); the four
+// 64 lanes folds by wave_max_sum (which leaves the same bits in all its lanes); the four
 // group results combine as max and (R0 + R1) + (R2 + R3). A workgroup of NT threads (NT
 // divides 256) runs virtual lanes tid, tid + NT, ...; the loads of four consecutive
 // partials of a lane are issued together. fold_groups leaves the group results in W[4],
@@ -102,10 +127,7 @@ __device__ __forceinline__ void fold_finish(const double2* __restrict__ part, ui
         r += p[j].y;
       }
     }
-    for (int o = 32; o > 0; o >>= 1) {
-      w = fmax(w, __shfl_xor(w, o, 64));
-      r += __shfl_xor(r, o, 64);
-    }
+    wave_max_sum(w, r);
     if ((v & 63u) == 0u) { W[v >> 6] = w; R[v >> 6] = r; }
   }
 }
@@ -176,13 +198,10 @@ __device__ __forceinline__ double block_max(double v, double* red) {
   }
   return m;  // valid on thread 0
 }
-// max of w and sum of r over the workgroup in one pass (one barrier); valid on thread 0
+// max of w (>= 0) and sum of r over the workgroup in one pass (one barrier); valid on thread 0
 template <int NT>
 __device__ __forceinline__ void block_max_sum(double& w, double& r, double* redW, double* redR) {
-  for (int o = 32; o > 0; o >>= 1) {
-    w = fmax(w, __shfl_xor(w, o, 64));
-    r += __shfl_xor(r, o, 64);
-  }
+  wave_max_sum(w, r);
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   if (lane == 0) { redW[wv] = w; redR[wv] = r; }
   __syncthreads();

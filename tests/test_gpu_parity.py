@@ -331,6 +331,32 @@ def test_integrals_match_host_sums(cold):
     assert I["mass"] > 0 and I["residual"] >= 0
 
 
+@pytest.mark.parametrize("name", ["hot_bell", "cold_cone"])
+def test_residual_matches_host(name):
+    """The residual of a step (the RMS over fluid groups of the host's density change per
+    unit grid time, over the chamber density; reduced on the device through per-tile
+    partials folded across lanes and waves) against the same sum over the read-back states
+    on the host: a fold that drops or doubles a lane, a row or a wave fails it."""
+    s = SimSetup(getattr(configs, name)(Nx=384))
+    g = GPUSolver()
+    load_setup(g, s)
+    g.step(60, s.P)
+    a0 = g.readAll()["A"].reshape(-1, 4)[:, 0].copy()
+    g.step(1, s.P)
+    a1 = g.readAll()["A"].reshape(-1, 4)[:, 0].copy()
+    dt = np.float32(g.readStepInfo()[0])
+    res = g.integrals(s.P)["residual"]
+    geoM = s.geoM.reshape(-1, 4)
+    host = (s.mask.reshape(-1) == 0) & (geoM[:, 0] == 0) & (geoM[:, 1] == 0)
+    dr = (a1[host] - a0[host]).astype(np.float64) / float(dt)
+    scale = float(np.float32(float(s.P.res[2]) * float(s.P.res[1]) / float(s.P.res[0])))
+    want = np.sqrt(np.sum(dr * dr) / host.sum()) * scale
+    say(f"[parity] {name} 384: residual device {res:.9e} host {want:.9e} ({host.sum()} groups)")
+    assert res > 0
+    assert res == pytest.approx(want, rel=1e-9)
+    g.close()
+
+
 def plane_integrals(s, A, B):
     """Exit-plane thrust and exhaust mass flow (simsetup.js:314-337), throat mass flow
     (simsetup.js:239-250) and chamber pressure at (iChamber, 0) (app.js:537) of a state,
