@@ -517,10 +517,17 @@ __global__ void __launch_bounds__(TX* TY, (STAGE == 2 && PHYS) ? 1 : W)
   // lanes with nothing to load discard the value: loads under a divergent branch would be
   // waited for inside it, one round trip per branch instead of one for the whole set.
   float4 gg = make_float4(0.f, 0.f, 0.f, 0.f);
+  // stage 2: the own cell's state at the start of the step (the second stage averages it
+  // in), loaded with the tile rather than at the update, where its round trip was exposed
+  R u0v[8];
   {
     const unsigned kc = (unsigned)(min(cj, Nr - 1) * Nx + min(ci, Nx - 1));
     const float4 g4 = gld(I.gg, kc);
     if (inDom) gg = g4;
+    if constexpr (STAGE == 2) {
+#pragma unroll
+      for (int m = 0; m < 8; ++m) u0v[m] = gld(U0 + m * Ns, kc);
+    }
   }
   // eddy-viscosity length scales of this thread's cells of the viscous ring tile
   constexpr int LPT = (S::TW * S::THh + NT - 1) / NT;
@@ -831,8 +838,8 @@ __global__ void __launch_bounds__(TX* TY, (STAGE == 2 && PHYS) ? 1 : W)
       if (STAGE == 1) {
         for (int m = 0; m < 4; ++m) { Uq[m] = a[m] + dt * invV * L[m]; UY[m] = b[m] + dt * invV * LB[m]; }
       } else {
-        R a0[4], b0[4];
-        for (int m = 0; m < 4; ++m) { a0[m] = gld(U0 + m * Ns, (unsigned)k); b0[m] = gld(U0 + (4 + m) * Ns, (unsigned)k); }
+        const R* a0 = u0v;
+        const R* b0 = u0v + 4;
         rho_old = a0[0];
         for (int m = 0; m < 4; ++m) {
           Uq[m] = R(0.5) * (a0[m] + a[m] + dt * invV * L[m]);
