@@ -365,11 +365,41 @@ __device__ __forceinline__ double muscl(double c, double a, double b) {
 template <typename R>
 __device__ __forceinline__ R jump_ratio(R a, R b) { return dv(fabs(a - b), fmin(a, b)); }
 
+// The MUSCL values of cell c in one direction, from its neighbours cm (behind) and cp
+// (ahead), in the face-normal frame (u normal): vp the value at its face ahead (the left
+// state there), vm the value at its face behind (the right state there), each formed as
+// muscl forms it in face_flux, so a face built from two cells' values is bit for bit the
+// face face_flux builds from the 4-cell stencil. A wall neighbour (r < 0) is the cell's
+// mirror image, its normal velocity reversed (solver.js:192-197). Order r, u, v, p, Z, F, G, O.
+template <typename R>
+__device__ __forceinline__ void muscl_values(Prim<R> cm, const Prim<R>& c, Prim<R> cp, R* vp, R* vm) {
+  if (cm.r < R(0)) { cm = c; cm.u = -cm.u; }
+  if (cp.r < R(0)) { cp = c; cp.u = -cp.u; }
+  const R cc[8] = {c.r, c.u, c.v, c.p, c.Z, c.F, c.G, c.O};
+  const R cb[8] = {cm.r, cm.u, cm.v, cm.p, cm.Z, cm.F, cm.G, cm.O};
+  const R ca[8] = {cp.r, cp.u, cp.v, cp.p, cp.Z, cp.F, cp.G, cp.O};
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    vp[q] = muscl<true>(cc[q], cc[q] - cb[q], ca[q] - cc[q]);
+    vm[q] = muscl<false>(cc[q], cc[q] - cb[q], ca[q] - cc[q]);
+  }
+}
+
+// The larger tangential pressure jump ratio at cell c, from its tangential neighbours a, b
+// (a wall neighbour counts as c's own pressure): the cell's part of the HLL blend weight
+// (solver.js:183-190).
+template <typename R>
+__device__ __forceinline__ R tang_jump(const Prim<R>& c, const Prim<R>& a, const Prim<R>& b) {
+  return fmax(jump_ratio(a.r < R(0) ? c.p : a.p, c.p), jump_ratio(b.r < R(0) ? c.p : b.p, c.p));
+}
+
 // Face flux: MUSCL van Leer reconstruction with positivity fallback, wall mirror
 // states, HLLC with Roe-averaged wave speeds blended to HLL by `hllw`
 // (solver.js:177-255). Inputs are in the face-normal frame (u normal).
 // Outputs F = (mass, normal mom, tangential mom, energy), FY = (rhoZ u, rhoF u,
 // uf*Gf, uf), FO = rhoO u.
+template <typename R>
+__device__ __forceinline__ void riemann_flux(const Prim<R>& WL, const Prim<R>& WR, R hllw, R* F, R* FY, R& FO);
 template <typename R>
 __device__ __forceinline__ void face_flux(Prim<R> c0, Prim<R> c1, Prim<R> c2, Prim<R> c3, R hllw, bool second,
                                           R* F, R* FY, R& FO) {
@@ -405,6 +435,37 @@ __device__ __forceinline__ void face_flux(Prim<R> c0, Prim<R> c1, Prim<R> c2, Pr
       }
     }
   }
+  riemann_flux(WL, WR, hllw, F, FY, FO);
+}
+
+// Face states of the face between cells c1 and c2 from their MUSCL values (muscl_values:
+// vL the left cell's value ahead, vR the right cell's value behind): as face_flux forms them.
+template <typename R>
+__device__ __forceinline__ void face_states(const Prim<R>& c1, const Prim<R>& c2, const R* vL, const R* vR, bool second,
+                                            Prim<R>& WL, Prim<R>& WR) {
+  const bool s1 = c1.r < R(0), s2 = c2.r < R(0);
+  if (s1) { WR = c2; WL = c2; WL.u = -WL.u; return; }
+  if (s2) { WL = c1; WR = c1; WR.u = -WR.u; return; }
+  WL = c1;
+  WR = c2;
+  if (second && vL[0] > R(0) && vL[3] > R(0) && vR[0] > R(0) && vR[3] > R(0)) {
+    WL.r = vL[0]; WL.u = vL[1]; WL.v = vL[2]; WL.p = vL[3];
+    WR.r = vR[0]; WR.u = vR[1]; WR.v = vR[2]; WR.p = vR[3];
+    WL.Z = clampR(vL[4], R(0), R(1));
+    WL.F = clampR(vL[5], R(0), R(1));
+    WL.G = clampR(vL[6], R(0.5), R(100));
+    WL.O = clampR(vL[7], R(0), R(1));
+    WR.Z = clampR(vR[4], R(0), R(1));
+    WR.F = clampR(vR[5], R(0), R(1));
+    WR.G = clampR(vR[6], R(0.5), R(100));
+    WR.O = clampR(vR[7], R(0), R(1));
+  }
+}
+
+// HLLC flux of the face states WL, WR with Roe-averaged wave speeds, blended to HLL by
+// `hllw` (solver.js:206-255), in the face-normal frame.
+template <typename R>
+__device__ __forceinline__ void riemann_flux(const Prim<R>& WL, const Prim<R>& WR, R hllw, R* F, R* FY, R& FO) {
   const R GL = WL.G, GR = WR.G;
   const R rL = WL.r, uL = WL.u, vL = WL.v, pL = WL.p;
   const R rR = WR.r, uR = WR.u, vR = WR.v, pR = WR.p;
@@ -479,12 +540,11 @@ __device__ __forceinline__ void face_flux(Prim<R> c0, Prim<R> c1, Prim<R> c2, Pr
   FY[3] = uf;
 }
 
-// HLL blending weight from tangential pressure jumps (solver.js:183-190);
-// pX is the pressure of a neighbour or the fall-back when it is a wall.
+// HLL blending weight of a face from its two cells' tangential jump ratios (tang_jump;
+// solver.js:183-190)
 template <typename R>
-__device__ __forceinline__ R hll_weight(R p1, R p2, R p1a, R p1b, R p2a, R p2b, R blend) {
-  const R jm = fmax(fmax(jump_ratio(p1a, p1), jump_ratio(p1b, p1)), fmax(jump_ratio(p2a, p2), jump_ratio(p2b, p2)));
-  return smoothstepR(R(0.15), R(0.6), jm * blend);
+__device__ __forceinline__ R hll_weight(R j1, R j2, R blend) {
+  return smoothstepR(R(0.15), R(0.6), fmax(j1, j2) * blend);
 }
 
 // reflected-Riemann pressure on a slip wall (solver.js:301-306)

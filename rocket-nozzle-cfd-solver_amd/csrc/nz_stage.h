@@ -56,6 +56,12 @@ __device__ __forceinline__ double dpp_move(double x, double id) {
   const int hi = __builtin_amdgcn_update_dpp(__double2hiint(id), __double2hiint(x), CTRL, ROW_MASK, 0xF, false);
   return __hiloint2double(hi, lo);
 }
+// The value of lane L - 1 of the wave (DPP wave_shr:1; lane 0 takes 0). Every lane of the
+// wave must be active.
+__device__ __forceinline__ float lane_prev(float x) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, x), 0x138, 0xF, 0xF, false));
+}
+__device__ __forceinline__ double lane_prev(double x) { return dpp_move<0x138, 0xF>(x, 0.0); }
 // Max of w (>= 0) and sum of r over the 64 lanes of a wave, left in every lane, in one fixed
 // order: swaps within quads (lane ^ 1, lane ^ 2), the half-row and row mirrors (each row of
 // 16 lanes then holds its row's result in every lane), row 0 into row 1 and row 2 into
@@ -251,8 +257,7 @@ __device__ __forceinline__ void face_flux_w(const DevInst& I, bool dirR_, int fi
     if (I.c.hllBlend > 0.f) {
       const Prim<R> a1 = getP(fi - dx + tx, fj - dy + ty), b1 = getP(fi - dx - tx, fj - dy - ty);
       const Prim<R> a2 = getP(fi + tx, fj + ty), b2 = getP(fi - tx, fj - ty);
-      const R w = hll_weight<R>(c1.p, c2.p, a1.r < R(0) ? c1.p : a1.p, b1.r < R(0) ? c1.p : b1.p,
-                                a2.r < R(0) ? c2.p : a2.p, b2.r < R(0) ? c2.p : b2.p, R(I.c.hllBlend));
+      const R w = hll_weight<R>(tang_jump(c1, a1, b1), tang_jump(c2, a2, b2), R(I.c.hllBlend));
       hllw = (s1 || s2) ? R(0) : w;
     }
     if (DIR == 1) {
@@ -285,8 +290,7 @@ __device__ __forceinline__ void face_flux_w(const DevInst& I, bool dirR_, int fi
   if (I.c.hllBlend > 0.f && !(s1 || s2)) {
     const Prim<R> a1 = getP(fi - dx + tx, fj - dy + ty), b1 = getP(fi - dx - tx, fj - dy - ty);
     const Prim<R> a2 = getP(fi + tx, fj + ty), b2 = getP(fi - tx, fj - ty);
-    hllw = hll_weight<R>(c1.p, c2.p, a1.r < R(0) ? c1.p : a1.p, b1.r < R(0) ? c1.p : b1.p,
-                         a2.r < R(0) ? c2.p : a2.p, b2.r < R(0) ? c2.p : b2.p, R(I.c.hllBlend));
+    hllw = hll_weight<R>(tang_jump(c1, a1, b1), tang_jump(c2, a2, b2), R(I.c.hllBlend));
   }
   if (dirR) {
     R t;
@@ -300,6 +304,32 @@ __device__ __forceinline__ void face_flux_w(const DevInst& I, bool dirR_, int fi
   const R a = R(fabsf(area));
   for (int q = 0; q < 4; ++q) { F[q] *= a; FY[q] *= a; }
   FO *= a;
+}
+
+// the primitives with the velocity components exchanged (the r-face-normal frame)
+template <typename R>
+__device__ __forceinline__ Prim<R> swap_uv(Prim<R> q) {
+  const R t = q.u; q.u = q.v; q.v = t;
+  return q;
+}
+
+// One tile face's inviscid flux times |area| from its two cells' primitives c1, c2, MUSCL
+// values vL (c1's value ahead) and vR (c2's value behind) and tangential jump ratios j1, j2
+// (formed once per cell, in the face-normal frame of direction DIR): the same flux, bit for
+// bit, as face_flux_w forms from the 4-cell stencil, and dropped likewise.
+template <typename R, int DIR>
+__device__ __forceinline__ void face_flux_pc(const DevInst& I, int fj, const Prim<R>& c1, const Prim<R>& c2, const R* vL,
+                                             const R* vR, R j1, R j2, float area, R* F, R* FY, R& FO) {
+  const bool s1 = c1.r < R(0), s2 = c2.r < R(0);
+  const bool keep = area != 0.f && !(DIR == 1 && fj == 0) && !(s1 && s2);
+  const R hllw = (I.c.hllBlend > 0.f && !(s1 || s2)) ? hll_weight<R>(j1, j2, R(I.c.hllBlend)) : R(0);
+  Prim<R> WL, WR;
+  face_states(c1, c2, vL, vR, I.c.order2 && area > 0.f, WL, WR);
+  riemann_flux(WL, WR, hllw, F, FY, FO);
+  if (DIR == 1) { const R t = F[1]; F[1] = F[2]; F[2] = t; }
+  const R a = R(fabsf(area));
+  for (int q = 0; q < 4; ++q) { F[q] = keep ? F[q] * a : R(0); FY[q] = keep ? FY[q] * a : R(0); }
+  FO = keep ? FO * a : R(0);
 }
 
 // One face's viscous flux times |area| (FS_VFLUX_X/R); getT(i, j, T1, T2) gives the
@@ -355,10 +385,15 @@ __device__ __forceinline__ float face_area(const DevInst& I, bool dirR, int fi, 
 }
 
 // ------------------------------------------------------------------------------
-template <typename R, int TX, int TY>
+template <typename R, int TX, int TY, bool PC>
 struct StageInvSmem {
   R Fx[9][TY][TX + 1];
   R Fr[9][TY + 1][TX];
+  // per-cell face states (PC): each cell's r-direction MUSCL value at its north face and
+  // x-direction tangential jump ratio, for the south face of the cell above it (none for
+  // the top row, whose north faces the boundary pass forms)
+  R TS[PC ? 8 : 1][PC ? TY - 1 : 1][PC ? TX : 1];
+  R JS[PC ? TY - 1 : 1][PC ? TX : 1];
 };
 template <typename R, int TX, int TY>
 struct StageVisSmem {
@@ -368,17 +403,17 @@ struct StageVisSmem {
 };
 // the viscous arrays share the inviscid flux arrays' LDS, and exist only in the kernels
 // that run the split step
-template <typename R, int TX, int TY, bool VIS>
+template <typename R, int TX, int TY, bool VIS, bool PC>
 union StageFluxSmem {
-  StageInvSmem<R, TX, TY> inv;
+  StageInvSmem<R, TX, TY, PC> inv;
 };
-template <typename R, int TX, int TY>
-union StageFluxSmem<R, TX, TY, true> {
-  StageInvSmem<R, TX, TY> inv;
+template <typename R, int TX, int TY, bool PC>
+union StageFluxSmem<R, TX, TY, true, PC> {
+  StageInvSmem<R, TX, TY, PC> inv;
   StageVisSmem<R, TX, TY> vis;
 };
 
-template <typename R, int TX, int TY, bool VIS>
+template <typename R, int TX, int TY, bool VIS, bool PC>
 struct StageSmem {
   static constexpr int PW = TX + 4, PH = TY + 4;   // primitive tile with 2-cell halo
   static constexpr int NFX = (TX + 1) * TY, NFR = TX * (TY + 1);
@@ -397,7 +432,7 @@ struct StageSmem {
   R P[8][PH][PW];
   float AX[TY][TX + 1];                            // signed areas of the tile's x faces
   float AR[TY + 1][TX];                            // ... and r faces (0 outside the domain)
-  StageFluxSmem<R, TX, TY, VIS> u;
+  StageFluxSmem<R, TX, TY, VIS, PC> u;
   static constexpr int MAXE = MAX_EXTRA_FACES;     // faces of merged cells outside the tile
   static constexpr int MAXF = MAX_FAR_CELLS;       // cells those faces reach beyond the halo
   static constexpr int WW = TX + 2 * FAR_HALO, WH = TY + 2 * FAR_HALO;
@@ -421,7 +456,11 @@ template <typename R, int TX, int TY, int STAGE, bool PHYS, int W>
 __global__ void __launch_bounds__(TX* TY, (STAGE == 2 && PHYS) ? 1 : W)
     k_stage(const DevInst* __restrict__ insts, int ninst, int chunk, const unsigned short* __restrict__ tileInst) {
   constexpr int NT = TX * TY;
-  using S = StageSmem<R, TX, TY, STAGE == 2 && PHYS>;
+  // per-cell face states (the latency-tuned shapes whose tile rows fit a wave): each cell's
+  // MUSCL values and tangential jump ratios are formed once, by its own thread, and the
+  // tile's west column and south row of faces move to the boundary pass
+  constexpr bool PC = W < 4 && TX <= 64;
+  using S = StageSmem<R, TX, TY, STAGE == 2 && PHYS, PC>;
   __shared__ S sm;
   const int tid = threadIdx.x;
   const int t = xcd_tile(blockIdx.x, gridDim.x, chunk);
@@ -675,7 +714,44 @@ __global__ void __launch_bounds__(TX* TY, (STAGE == 2 && PHYS) ? 1 : W)
       for (int q = 0; q < 4; ++q) { sm.u.inv.Fr[q][fy][fx] = F[q]; sm.u.inv.Fr[4 + q][fy][fx] = FY[q]; }
       sm.u.inv.Fr[8][fy][fx] = FO;
     };
-    {
+    if constexpr (PC) {
+      // each cell's MUSCL values in both directions and its tangential jump ratios; the
+      // x-direction value at its east face reaches the west face of the next cell from the
+      // neighbouring lane (the tile's west column of faces is the boundary pass's), the
+      // r-direction value at its north face the south face of the cell above through LDS
+      // (the south row likewise)
+      const int li = tx + 2, lj = ty + 2;
+      const Prim<R> cc = P_at(li, lj), cw = P_at(li - 1, lj), ce = P_at(li + 1, lj);
+      const Prim<R> cs = P_at(li, lj - 1), cn = P_at(li, lj + 1);
+      const bool blend = I.c.hllBlend > 0.f;
+      R pX[8], mX[8], pY[8], mY[8];
+      muscl_values(cw, cc, ce, pX, mX);
+      muscl_values(swap_uv(cs), swap_uv(cc), swap_uv(cn), pY, mY);
+      const R jY = blend ? tang_jump(cc, cn, cs) : R(0);  // x faces: tangential neighbours in r
+      const R jX = blend ? tang_jump(cc, ce, cw) : R(0);  // r faces: tangential neighbours in x
+      if (ty < TY - 1) {
+        for (int q = 0; q < 8; ++q) sm.u.inv.TS[q][ty][tx] = pY[q];
+        sm.u.inv.JS[ty][tx] = jX;
+      }
+      {
+        R vW[8];
+        for (int q = 0; q < 8; ++q) vW[q] = lane_prev(pX[q]);
+        const R jW = lane_prev(jY);
+        R F[4], FY[4], FO;
+        face_flux_pc<R, 0>(I, j0 + ty, cw, cc, vW, mX, jW, jY, sm.AX[ty][tx], F, FY, FO);
+        if (tx > 0) putX(tx, ty, F, FY, FO);
+      }
+      __syncthreads();
+      {
+        const int tb = ty > 0 ? ty - 1 : 0;
+        R vS[8];
+        for (int q = 0; q < 8; ++q) vS[q] = sm.u.inv.TS[q][tb][tx];
+        const R jS = sm.u.inv.JS[tb][tx];
+        R F[4], FY[4], FO;
+        face_flux_pc<R, 1>(I, j0 + ty, swap_uv(cs), swap_uv(cc), vS, mY, jS, jX, sm.AR[ty][tx], F, FY, FO);
+        if (ty > 0) putR(tx, ty, F, FY, FO);
+      }
+    } else {
       R F[4], FY[4], FO;
       face_flux_w<R, 0>(I, false, i0 + tx, j0 + ty, tileP, tileA, F, FY, FO);
       putX(tx, ty, F, FY, FO);
@@ -691,14 +767,17 @@ __global__ void __launch_bounds__(TX* TY, (STAGE == 2 && PHYS) ? 1 : W)
     // in turn) pack the tile's faces into as few waves as possible and evaluate them
     // with the one routine that takes the direction at run time.
     constexpr bool SPLIT = W < 4;
-    constexpr int B_E = SPLIT ? (TX + 63) / 64 * 64 : TX, B_X = B_E + (SPLIT ? (TY + 63) / 64 * 64 : TY);
+    constexpr int NB = PC ? 2 : 1;  // with per-cell face states also the south row and west column
+    constexpr int B_E = SPLIT ? (NB * TX + 63) / 64 * 64 : TX, B_X = B_E + (SPLIT ? (NB * TY + 63) / 64 * 64 : TY);
     for (int v = tid; v < B_X + nEx; v += NT) {
       R F[4], FY[4], FO;
       if (v < B_X) {
         const bool dirR = v < B_E;
         const int k = dirR ? v : v - B_E;
-        if (k < (dirR ? TX : TY)) {
-          const int fx = dirR ? k : TX, fy = dirR ? TY : k;
+        if (k < NB * (dirR ? TX : TY)) {
+          const bool sw = k >= (dirR ? TX : TY);  // the south row / west column
+          const int kk = sw ? k - (dirR ? TX : TY) : k;
+          const int fx = dirR ? kk : (sw ? 0 : TX), fy = dirR ? (sw ? 0 : TY) : kk;
           if constexpr (SPLIT) {
             if (dirR) face_flux_w<R, 1>(I, true, i0 + fx, j0 + fy, tileP, tileA, F, FY, FO);
             else face_flux_w<R, 0>(I, false, i0 + fx, j0 + fy, tileP, tileA, F, FY, FO);
@@ -964,11 +1043,13 @@ __global__ void __launch_bounds__(TX* TY, (STAGE == 2 && PHYS) ? 1 : W)
           vface_w<R, 1>(I, true, i0 + tx, j0 + ty, tileP, tileT, tileA, F, FY);
           putVR(tx, ty, F, FY);
         }
-        for (int v = tid; v < B_X + nEx; v += NT) {
+        // the north row, east column and extra faces, laid out as in the inviscid pass
+        constexpr int VB_E = SPLIT ? (TX + 63) / 64 * 64 : TX, VB_X = VB_E + (SPLIT ? (TY + 63) / 64 * 64 : TY);
+        for (int v = tid; v < VB_X + nEx; v += NT) {
           R F[4], FY[3];
-          if (v < B_X) {
-            const bool dirR = v < B_E;
-            const int k = dirR ? v : v - B_E;
+          if (v < VB_X) {
+            const bool dirR = v < VB_E;
+            const int k = dirR ? v : v - VB_E;
             if (k < (dirR ? TX : TY)) {
               const int fx = dirR ? k : TX, fy = dirR ? TY : k;
               if constexpr (SPLIT) {
@@ -981,7 +1062,7 @@ __global__ void __launch_bounds__(TX* TY, (STAGE == 2 && PHYS) ? 1 : W)
               else putVX(fx, fy, F, FY);
             }
           } else {
-            const int e = v - B_X;
+            const int e = v - VB_X;
             const uint2 ef = sm.ex[e];
             vface_w<R>(I, ex_dir(ef), ex_i(ef), ex_j(ef), anyP, anyT, globA, F, FY);
             for (int q = 0; q < 4; ++q) gst(exFlux, e * 16 + 9 + q, F[q]);
