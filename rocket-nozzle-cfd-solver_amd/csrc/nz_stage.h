@@ -385,15 +385,19 @@ __device__ __forceinline__ float face_area(const DevInst& I, bool dirR, int fi, 
 }
 
 // ------------------------------------------------------------------------------
-template <typename R, int TX, int TY, bool PC>
+template <typename R, int TX, int TY>
 struct StageInvSmem {
   R Fx[9][TY][TX + 1];
-  R Fr[9][TY + 1][TX];
-  // per-cell face states (PC): each cell's r-direction MUSCL value at its north face and
-  // x-direction tangential jump ratio, for the south face of the cell above it (none for
-  // the top row, whose north faces the boundary pass forms)
-  R TS[PC ? 8 : 1][PC ? TY - 1 : 1][PC ? TX : 1];
-  R JS[PC ? TY - 1 : 1][PC ? TX : 1];
+  union {
+    R Fr[9][TY + 1][TX];
+    // per-cell face states, until the r faces are formed: each cell's r-direction MUSCL
+    // value at its north face and x-direction tangential jump ratio, for the south face of
+    // the cell above it (none for the top row, whose north faces the boundary pass forms)
+    struct {
+      R TS[8][TY - 1][TX];
+      R JS[TY - 1][TX];
+    } pc;
+  };
 };
 template <typename R, int TX, int TY>
 struct StageVisSmem {
@@ -403,17 +407,17 @@ struct StageVisSmem {
 };
 // the viscous arrays share the inviscid flux arrays' LDS, and exist only in the kernels
 // that run the split step
-template <typename R, int TX, int TY, bool VIS, bool PC>
+template <typename R, int TX, int TY, bool VIS>
 union StageFluxSmem {
-  StageInvSmem<R, TX, TY, PC> inv;
+  StageInvSmem<R, TX, TY> inv;
 };
-template <typename R, int TX, int TY, bool PC>
-union StageFluxSmem<R, TX, TY, true, PC> {
-  StageInvSmem<R, TX, TY, PC> inv;
+template <typename R, int TX, int TY>
+union StageFluxSmem<R, TX, TY, true> {
+  StageInvSmem<R, TX, TY> inv;
   StageVisSmem<R, TX, TY> vis;
 };
 
-template <typename R, int TX, int TY, bool VIS, bool PC>
+template <typename R, int TX, int TY, bool VIS>
 struct StageSmem {
   static constexpr int PW = TX + 4, PH = TY + 4;   // primitive tile with 2-cell halo
   static constexpr int NFX = (TX + 1) * TY, NFR = TX * (TY + 1);
@@ -432,7 +436,7 @@ struct StageSmem {
   R P[8][PH][PW];
   float AX[TY][TX + 1];                            // signed areas of the tile's x faces
   float AR[TY + 1][TX];                            // ... and r faces (0 outside the domain)
-  StageFluxSmem<R, TX, TY, VIS, PC> u;
+  StageFluxSmem<R, TX, TY, VIS> u;
   static constexpr int MAXE = MAX_EXTRA_FACES;     // faces of merged cells outside the tile
   static constexpr int MAXF = MAX_FAR_CELLS;       // cells those faces reach beyond the halo
   static constexpr int WW = TX + 2 * FAR_HALO, WH = TY + 2 * FAR_HALO;
@@ -456,11 +460,14 @@ template <typename R, int TX, int TY, int STAGE, bool PHYS, int W>
 __global__ void __launch_bounds__(TX* TY, (STAGE == 2 && PHYS) ? 1 : W)
     k_stage(const DevInst* __restrict__ insts, int ninst, int chunk, const unsigned short* __restrict__ tileInst) {
   constexpr int NT = TX * TY;
-  // per-cell face states (the latency-tuned shapes whose tile rows fit a wave): each cell's
-  // MUSCL values and tangential jump ratios are formed once, by its own thread, and the
-  // tile's west column and south row of faces move to the boundary pass
-  constexpr bool PC = W < 4 && TX <= 64;
-  using S = StageSmem<R, TX, TY, STAGE == 2 && PHYS, PC>;
+  // per-cell face states (shapes whose tile rows fit a wave): each cell's MUSCL values and
+  // tangential jump ratios are formed once, by its own thread, and the tile's west column
+  // and south row of faces move to the boundary pass. The throughput shapes pack that pass
+  // into as few waves as possible: there the moved faces are free only while all four
+  // sides still fit one wave (2 (TX + TY) <= 64); otherwise they cost a wave more than the
+  // per-cell values save, and those shapes keep the 4-cell stencil per face.
+  constexpr bool PC = TX <= 64 && (W < 4 || 2 * (TX + TY) <= 64);
+  using S = StageSmem<R, TX, TY, STAGE == 2 && PHYS>;
   __shared__ S sm;
   const int tid = threadIdx.x;
   const int t = xcd_tile(blockIdx.x, gridDim.x, chunk);
@@ -730,8 +737,8 @@ __global__ void __launch_bounds__(TX* TY, (STAGE == 2 && PHYS) ? 1 : W)
       const R jY = blend ? tang_jump(cc, cn, cs) : R(0);  // x faces: tangential neighbours in r
       const R jX = blend ? tang_jump(cc, ce, cw) : R(0);  // r faces: tangential neighbours in x
       if (ty < TY - 1) {
-        for (int q = 0; q < 8; ++q) sm.u.inv.TS[q][ty][tx] = pY[q];
-        sm.u.inv.JS[ty][tx] = jX;
+        for (int q = 0; q < 8; ++q) sm.u.inv.pc.TS[q][ty][tx] = pY[q];
+        sm.u.inv.pc.JS[ty][tx] = jX;
       }
       {
         R vW[8];
@@ -742,11 +749,13 @@ __global__ void __launch_bounds__(TX* TY, (STAGE == 2 && PHYS) ? 1 : W)
         if (tx > 0) putX(tx, ty, F, FY, FO);
       }
       __syncthreads();
+      // the south neighbour's values, read before any r-face flux overwrites them
+      const int tb = ty > 0 ? ty - 1 : 0;
+      R vS[8];
+      for (int q = 0; q < 8; ++q) vS[q] = sm.u.inv.pc.TS[q][tb][tx];
+      const R jS = sm.u.inv.pc.JS[tb][tx];
+      __syncthreads();
       {
-        const int tb = ty > 0 ? ty - 1 : 0;
-        R vS[8];
-        for (int q = 0; q < 8; ++q) vS[q] = sm.u.inv.TS[q][tb][tx];
-        const R jS = sm.u.inv.JS[tb][tx];
         R F[4], FY[4], FO;
         face_flux_pc<R, 1>(I, j0 + ty, swap_uv(cs), swap_uv(cc), vS, mY, jS, jX, sm.AR[ty][tx], F, FY, FO);
         if (ty > 0) putR(tx, ty, F, FY, FO);
@@ -758,17 +767,18 @@ __global__ void __launch_bounds__(TX* TY, (STAGE == 2 && PHYS) ? 1 : W)
       face_flux_w<R, 1>(I, true, i0 + tx, j0 + ty, tileP, tileA, F, FY, FO);
       putR(tx, ty, F, FY, FO);
     }
-    // The tile's north row of r faces, its east column of x faces and the extra faces.
-    // Shapes tuned for latency (W < 4: one wave of workgroups fills the GPU) start each
-    // kind on a wave of its own (work item v: [0, TX) north faces, from B_E the east
-    // faces, from B_X the extra faces), so the kinds run side by side on different SIMDs
-    // rather than one after another in one divergent wave, each with the flux routine
-    // specialised for its direction; the throughput shapes (W = 4, many workgroups per CU
-    // in turn) pack the tile's faces into as few waves as possible and evaluate them
-    // with the one routine that takes the direction at run time.
+    // The tile's north row of r faces and east column of x faces (with per-cell face
+    // states also its south row and west column), then the extra faces. Work item v:
+    // [0, TX) north faces, [TX, 2 TX) south faces, from B_E the east faces and then the
+    // west faces, from B_X the extra faces. Shapes tuned for latency (W < 4: one wave of
+    // workgroups fills the GPU) start each kind on a wave of its own, so the kinds run side
+    // by side on different SIMDs rather than one after another in one divergent wave, each
+    // with the flux routine specialised for its direction; the throughput shapes (W = 4,
+    // many workgroups per CU in turn) pack the tile's faces into as few waves as possible
+    // and evaluate them with the one routine that takes the direction at run time.
     constexpr bool SPLIT = W < 4;
     constexpr int NB = PC ? 2 : 1;  // with per-cell face states also the south row and west column
-    constexpr int B_E = SPLIT ? (NB * TX + 63) / 64 * 64 : TX, B_X = B_E + (SPLIT ? (NB * TY + 63) / 64 * 64 : TY);
+    constexpr int B_E = SPLIT ? (NB * TX + 63) / 64 * 64 : NB * TX, B_X = B_E + (SPLIT ? (NB * TY + 63) / 64 * 64 : NB * TY);
     for (int v = tid; v < B_X + nEx; v += NT) {
       R F[4], FY[4], FO;
       if (v < B_X) {
