@@ -50,6 +50,21 @@ def test_every_shape_bit_identical(name, precision):
         assert np.array_equal(out["A"], ref["A"]) and np.array_equal(out["B"], ref["B"]), shapes[k]
 
 
+@pytest.mark.parametrize("order2,hll", [(0, 1.0), (1, 0.0)])
+def test_every_shape_bit_identical_scheme_options(order2, hll):
+    # first-order faces (the per-cell MUSCL values formed but not taken) and no HLL blend
+    # (the jump ratios not formed): the per-cell and the stencil face forms still agree
+    s = SimSetup(configs.hot_bell(Nx=192))
+    s.P.order2 = order2
+    s.P.hllBlend = hll
+    shapes = tile_shapes(32)
+    ref = run(s, 40, 0)
+    for k in range(1, len(shapes)):
+        say(f"[shapes] hot_bell order2={order2} hllBlend={hll} shape {k} {shapes[k]}")
+        out = run(s, 40, k)
+        assert np.array_equal(out["A"], ref["A"]) and np.array_equal(out["B"], ref["B"]), shapes[k]
+
+
 def test_autotune_leaves_state_and_picks_a_shape():
     s = SimSetup(configs.hot_bell(Nx=384))
     g = GPUSolver()
