@@ -38,9 +38,19 @@ def run(setup, steps, shape, precision=32):
     return g.readAllF64() if precision == 64 else g.readAll()
 
 
-@pytest.mark.parametrize("name,precision", [("hot_bell", 32), ("cold_cone", 32), ("hot_bell", 64)])
+@pytest.mark.parametrize("name,precision", [("hot_bell", 32), ("cold_cone", 32), ("hot_bell", 64), ("hot_bell_200", 32),
+                                            ("kerolox_injector", 32)])
 def test_every_shape_bit_identical(name, precision):
-    s = SimSetup(configs.hot_bell(Nx=192) if name == "hot_bell" else configs.cold_cone())
+    # hot_bell_200 (200 x 67) and the injector (160 x 53) leave partial tiles at the grid's
+    # east and north edges for most shapes
+    if name == "hot_bell":
+        s = SimSetup(configs.hot_bell(Nx=192))
+    elif name == "hot_bell_200":
+        s = SimSetup(configs.hot_bell(Nx=200))
+    elif name == "kerolox_injector":
+        s = SimSetup(configs.kerolox_injector(Nx=160))
+    else:
+        s = SimSetup(configs.cold_cone())
     shapes = tile_shapes(precision)
     assert len(shapes) >= 3 and shapes[0] in ((64, 4), (32, 4))
     ref = run(s, 40, 0, precision)
